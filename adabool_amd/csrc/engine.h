@@ -1,0 +1,115 @@
+// engine.h -- host-side engine objects shared by the C ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+
+namespace ab {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    ab_status ensure(size_t bytes);
+    void release();
+};
+
+// Grow-only pinned host buffer.
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    ab_status ensure(size_t bytes);
+    void release();
+};
+
+// Arguments of one batched evaluation.
+struct BatchArgs {
+    const uint32_t *d_keys = nullptr;  // concatenated keys (device)
+    const uint64_t *h_qoff = nullptr;  // nq+1 offsets (host)
+    uint32_t nq = 0;
+    int path = AB_PATH_THRESHOLD;
+    const uint8_t *h_sel = nullptr;   // per-query path decided on the host, or NULL
+    const uint64_t *h_M = nullptr;    // per-query matched records with h_sel, or NULL
+    uint32_t *d_bitmaps = nullptr;    // NULL: context scratch
+    uint64_t out_stride = 0;          // u32 words between consecutive queries' bitmaps
+    uint64_t out_words = 0;           // words written per query
+    uint32_t *d_ids = nullptr;
+    uint64_t ids_stride = 0;
+    uint32_t *d_counts = nullptr;
+    uint8_t *h_paths = nullptr;
+};
+
+// MLP cost model: 12 -> 64 -> 64 -> 2, FP32, standardized inputs.
+constexpr int kF = AB_NFEATURES, kH1 = 64, kH2 = 64, kO = 2;
+constexpr int kParams = kH1 * kF + kH1 + kH2 * kH1 + kH2 + kO * kH2 + kO;
+
+}  // namespace ab
+
+struct ab_index {
+    uint32_t nkeys = 0, nvalues = 0;
+    uint64_t nnz = 0;
+    uint64_t nrec[AB_NTYPES] = {0};
+    uint64_t R = 0, list_bytes = 0;
+    uint32_t tile_shift = 0, ntiles = 0;
+    uint64_t nblocks = 0, kbytes = 0, stride = 0;
+    std::vector<uint32_t> seg8;  // host copy for host-side dispatch
+    std::vector<void *> allocs;
+    uint64_t dev_bytes = 0;
+    ab::DevIndex dev{};
+    int device = 0;
+    bool host_only = false;
+    ab::HostIndex host;  // full host layout (host-only handles)
+};
+
+struct ab_ctx {
+    ab_index *ix = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double tau = 0.05;
+    uint32_t group = 64;
+    uint32_t dense_threads = 256;
+    uint32_t sparse_chunk = 4096;
+    // device scratch
+    ab::DevBuf qoff, sel, kq, cnt, koff, parts, feat, M, ccnt, bm, qlist, keys, qbm, mask;
+    // pinned staging: control data (qoff, decisions, query lists) and keys
+    ab::PinBuf hstage, hkstage, hsel, hM;
+    hipEvent_t stage_ev = nullptr, kstage_ev = nullptr;
+    bool stage_pending = false, kstage_pending = false;
+    // per-kernel-class timing
+    bool timing = false;
+    struct Span {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Span> spans;
+    std::vector<hipEvent_t> ev_pool;
+    double t_ms[AB_K_NCLASSES] = {0};
+    uint64_t t_n[AB_K_NCLASSES] = {0};
+    // cost model
+    bool mlp_loaded = false;
+    std::vector<float> mlp_host;  // params followed by mean[12], std[12]
+    ab::DevBuf mlp_params, mlp_norm, mlp_pred;
+    ~ab_ctx();
+};
+
+namespace ab {
+// Records a start event at construction and an end event at destruction on
+// the context stream when timing is on.
+struct TimedSpan {
+    ab_ctx *c;
+    int kind;
+    hipEvent_t a = nullptr;
+    TimedSpan(ab_ctx *ctx, int k);
+    ~TimedSpan();
+};
+ab_status run_batch(ab_ctx *c, const BatchArgs &a);
+// Device decision for nq queries from device features (MLP forward kernel).
+ab_status mlp_decide(ab_ctx *c, const float *d_feat, uint32_t nq, uint8_t *d_sel);
+// Host decision for one query's features.
+int mlp_decide_host(const ab_ctx *c, const float *f);
+}  // namespace ab

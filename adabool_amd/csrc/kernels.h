@@ -1,0 +1,65 @@
+// kernels.h -- launch wrappers of the AdaBool CDNA4 kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ab {
+
+// Device view of a resident index (both layouts), passed to kernels by value.
+struct DevIndex {
+    const uint32_t *seg8;      // (nkeys+1) * 8 prefix rows
+    const uint32_t *hdr[6];    // key-major headers per type
+    const uint8_t *pay[6];     // key-major payloads per type
+    const uint32_t *vhdr[6];   // value-tiled headers per type
+    const uint8_t *vpay[6];    // value-tiled payloads per type
+    const uint4 *vblk;         // BlockMeta per key-stream block
+    const uint8_t *vkeys;      // transposed delta bytes
+    const uint32_t *tile_blk;  // ntiles+1 block prefix
+    uint32_t nkeys, nvalues, tile_shift, ntiles;
+};
+
+constexpr int kMaxGroup = 64;
+struct DenseGroup {
+    uint32_t B;             // queries in the group (1..64)
+    uint32_t q[kMaxGroup];  // output slot (query index) of each group member
+};
+
+// mode of the dense kernel's key test
+enum DenseMode { kModeBitmap = 0, kModeMask32 = 1, kModeMask64 = 2 };
+
+// scan scratch: parts needed for n items
+uint64_t scan_parts_needed(uint64_t n);
+
+void launch_fill_zero(void *p, uint64_t bytes, hipStream_t s);
+void launch_zero_qlist(uint32_t *bm, uint64_t stride_words, uint64_t words, const uint32_t *qlist,
+                       uint32_t n, hipStream_t s);
+void launch_sparse_plan(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff,
+                        const uint8_t *sel, uint32_t nq, uint32_t *kq, uint64_t *cnt, hipStream_t s);
+void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *parts, hipStream_t s);
+void launch_sparse_union(const DevIndex &ix, const uint32_t *keys, const uint32_t *kq,
+                         const uint64_t *koff, uint64_t K, uint32_t *out, uint64_t out_stride,
+                         uint32_t chunk, uint32_t grid, hipStream_t s);
+// Sets bit b of mask[k] (or of bitmap word k>>5 for B==1 with bitmap mode)
+// for every key k of query g.q[b]; clear=1 writes zeros at the same places.
+void launch_mask_keys(int mode, const uint32_t *keys, const uint64_t *qoff, const DenseGroup &g,
+                      void *mask, uint32_t nkeys, int clear, hipStream_t s);
+void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_t b0, uint32_t B,
+                              void *mask, uint32_t nkeys, hipStream_t s);
+// dynamic LDS bytes of the dense kernel for a group of B queries
+uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift);
+hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask,
+                              const DenseGroup &g, uint32_t *out, uint64_t out_stride,
+                              uint64_t out_words, hipStream_t s);
+constexpr uint32_t kCompactWords = 2048;
+void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
+                     uint32_t *ccnt, hipStream_t s);
+void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
+                      const uint32_t *ccnt, uint32_t *ids, uint64_t ids_stride, uint32_t *counts,
+                      hipStream_t s);
+void launch_features(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff, uint32_t nq,
+                     uint64_t R, float *feats, uint64_t *M, hipStream_t s);
+void launch_decide_threshold(const uint64_t *M, uint32_t nq, double thr, uint8_t *sel, hipStream_t s);
+
+}  // namespace ab

@@ -1,0 +1,777 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the AdaBool OR-filtering path.
+//
+// Record emission follows ProcessArrayFunctor (src/AdaBool/inverted_list.cu.h:
+// 61-94) and the bitset emit (src/AdaBool/key_transformer.cu.h:100-111):
+// header value always; payload bytes [0, kMinNum) always, later bytes until the
+// first zero; a bitset ORs its 8 words into window words (header >> 5) + j.
+// Instead of one atomic per value, the values of a record (sorted, all in one
+// 256-value window) are folded into per-word bit masks and each touched word
+// is OR-ed once.
+#include <hip/hip_runtime.h>
+
+#include "adabool.h"
+#include "kernels.h"
+
+namespace ab {
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_incl_scan64(unsigned long long x)
+{
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        unsigned long long y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+template <int T> struct TypeTraits;
+template <> struct TypeTraits<AB_ARRAY1>  { static constexpr int P = 0;  static constexpr int kMin = 0; };
+template <> struct TypeTraits<AB_ARRAY4>  { static constexpr int P = 4;  static constexpr int kMin = 1; };
+template <> struct TypeTraits<AB_ARRAY8>  { static constexpr int P = 8;  static constexpr int kMin = 5; };
+template <> struct TypeTraits<AB_ARRAY16> { static constexpr int P = 16; static constexpr int kMin = 9; };
+template <> struct TypeTraits<AB_ARRAY32> { static constexpr int P = 32; static constexpr int kMin = 17; };
+template <> struct TypeTraits<AB_BITSET>  { static constexpr int P = 32; static constexpr int kMin = 0; };
+
+// Loads a record's payload as little-endian words (one 4/8/16-byte load, or two
+// 16-byte loads for 32-byte payloads).
+template <int T>
+__device__ __forceinline__ void load_payload(const uint8_t *__restrict__ pay, uint32_t idx, uint32_t (&w)[8])
+{
+    if constexpr (T == AB_ARRAY4) {
+        w[0] = reinterpret_cast<const uint32_t *>(pay)[idx];
+    } else if constexpr (T == AB_ARRAY8) {
+        uint2 v = reinterpret_cast<const uint2 *>(pay)[idx];
+        w[0] = v.x; w[1] = v.y;
+    } else if constexpr (T == AB_ARRAY16) {
+        uint4 v = reinterpret_cast<const uint4 *>(pay)[idx];
+        w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+    } else if constexpr (T == AB_ARRAY32 || T == AB_BITSET) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(pay) + 2ull * idx;
+        uint4 a = p[0], b = p[1];
+        w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+        w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+    }
+}
+
+// Calls flush(word, bits) once per result word the record touches.
+template <int T, class F>
+__device__ __forceinline__ void emit_record(uint32_t h, const uint32_t (&w)[8], F &&flush)
+{
+    if constexpr (T == AB_BITSET) {
+        const uint32_t base = h >> 5;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (w[j]) flush(base + j, w[j]);
+    } else if constexpr (T == AB_ARRAY1) {
+        flush(h >> 5, 1u << (h & 31));
+    } else {
+        constexpr int P = TypeTraits<T>::P, kMin = TypeTraits<T>::kMin;
+        const uint32_t hi = h & 0xffffff00u;
+        uint32_t cw = h >> 5, bits = 1u << (h & 31);
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const uint32_t b = (w[i >> 2] >> (8 * (i & 3))) & 0xffu;
+            if (i >= kMin && b == 0) break;
+            const uint32_t v = hi | b;
+            if ((v >> 5) != cw) {
+                flush(cw, bits);
+                cw = v >> 5;
+                bits = 0;
+            }
+            bits |= 1u << (v & 31);
+        }
+        flush(cw, bits);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fills
+// ---------------------------------------------------------------------------
+__global__ void k_fill_zero(uint4 *p, uint64_t n16)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n16;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(0, 0, 0, 0);
+}
+
+__global__ void k_fill_zero_tail(uint8_t *p, uint64_t n)
+{
+    uint64_t i = threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void launch_fill_zero(void *p, uint64_t bytes, hipStream_t s)
+{
+    if (!bytes) return;
+    uint64_t n16 = bytes / 16;
+    if (n16) {
+        uint64_t grid = (n16 + 255) / 256;
+        if (grid > 4096) grid = 4096;
+        hipLaunchKernelGGL(k_fill_zero, dim3((uint32_t)grid), dim3(256), 0, s, (uint4 *)p, n16);
+    }
+    if (bytes % 16)
+        hipLaunchKernelGGL(k_fill_zero_tail, dim3(1), dim3(64), 0, s, (uint8_t *)p + n16 * 16, bytes % 16);
+}
+
+constexpr uint32_t kZeroChunk = 8192;  // words per work-group
+__global__ __launch_bounds__(256) void k_zero_qlist(uint32_t *bm, uint64_t stride, uint64_t words,
+                                                    const uint32_t *qlist)
+{
+    const uint32_t q = qlist[blockIdx.y];
+    const uint64_t w0 = (uint64_t)blockIdx.x * kZeroChunk;
+    uint32_t *dst = bm + q * stride;
+    const uint64_t w1 = w0 + kZeroChunk < words ? w0 + kZeroChunk : words;
+    for (uint64_t i = w0 + threadIdx.x; i < w1; i += 256) dst[i] = 0;
+}
+
+void launch_zero_qlist(uint32_t *bm, uint64_t stride, uint64_t words, const uint32_t *qlist, uint32_t n,
+                       hipStream_t s)
+{
+    if (!n || !words) return;
+    uint32_t gx = (uint32_t)((words + kZeroChunk - 1) / kZeroChunk);
+    hipLaunchKernelGGL(k_zero_qlist, dim3(gx, n), dim3(256), 0, s, bm, stride, words, qlist);
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of u64 counts: out[0..n] (out[n] = total)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kScanNT = 256, kScanIPT = 16, kScanTile = kScanNT * kScanIPT;
+
+uint64_t scan_parts_needed(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+// Block-wide exclusive scan of one value per thread; *total gets the sum.
+__device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long x, unsigned long long *sh,
+                                                              unsigned long long *total)
+{
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    unsigned long long incl = wave_incl_scan64(x);
+    if (lane == 63) sh[wave] = incl;
+    __syncthreads();
+    unsigned long long before = 0, all = 0;
+    for (uint32_t w = 0; w < nw; ++w) {
+        if (w < wave) before += sh[w];
+        all += sh[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - x;
+}
+
+__global__ __launch_bounds__(kScanNT) void k_scan_tile_sums(const uint64_t *in, uint64_t n, uint64_t *parts)
+{
+    __shared__ unsigned long long sh[kScanNT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    unsigned long long s = 0;
+    for (uint32_t i = 0; i < kScanIPT; ++i) {
+        uint64_t j = base + (uint64_t)i * kScanNT + threadIdx.x;
+        if (j < n) s += in[j];
+    }
+    unsigned long long tot;
+    block_excl_scan(s, sh, &tot);
+    if (threadIdx.x == 0) parts[blockIdx.x] = tot;
+}
+
+// single work-group: exclusive scan of parts[0..np) in place; parts[np] = total
+__global__ __launch_bounds__(kScanNT) void k_scan_parts(uint64_t *parts, uint64_t np)
+{
+    __shared__ unsigned long long sh[kScanNT / 64];
+    unsigned long long carry = 0;
+    for (uint64_t base = 0; base < np; base += kScanNT) {
+        uint64_t j = base + threadIdx.x;
+        unsigned long long x = j < np ? parts[j] : 0, tot;
+        unsigned long long e = block_excl_scan(x, sh, &tot);
+        if (j < np) parts[j] = carry + e;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) parts[np] = carry;
+}
+
+__global__ __launch_bounds__(kScanNT) void k_scan_apply(const uint64_t *in, uint64_t n, const uint64_t *parts,
+                                                         uint64_t np, uint64_t *out)
+{
+    __shared__ unsigned long long sh[kScanNT / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanIPT;
+    unsigned long long v[kScanIPT], s = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kScanIPT; ++i) {
+        uint64_t j = base + i;
+        v[i] = j < n ? in[j] : 0;
+        s += v[i];
+    }
+    unsigned long long tot;
+    unsigned long long e = block_excl_scan(s, sh, &tot) + parts[blockIdx.x];
+#pragma unroll
+    for (uint32_t i = 0; i < kScanIPT; ++i) {
+        uint64_t j = base + i;
+        if (j < n) out[j] = e;
+        e += v[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = parts[np];
+}
+
+void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *parts, hipStream_t s)
+{
+    uint64_t np = (n + kScanTile - 1) / kScanTile;
+    if (np == 0) {
+        // out[0] = 0
+        launch_fill_zero(out, 8, s);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3((uint32_t)np), dim3(kScanNT), 0, s, in, n, parts);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanNT), 0, s, parts, np);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)np), dim3(kScanNT), 0, s, in, n, (const uint64_t *)parts, np,
+                       out);
+}
+
+// ---------------------------------------------------------------------------
+// sparse path: plan + load-balanced list union (key-major layout)
+// ---------------------------------------------------------------------------
+// Per key j of query q: cnt[j] = records of key j (all types) if q takes the
+// sparse path and the key is in range (Prepare(keys) extents,
+// src/AdaBool/inverted_list.h:491-505), else 0.
+__global__ __launch_bounds__(256) void k_sparse_plan(DevIndex ix, const uint32_t *__restrict__ keys,
+                                                     const uint64_t *__restrict__ qoff,
+                                                     const uint8_t *__restrict__ sel, uint32_t *kq, uint64_t *cnt)
+{
+    const uint32_t q = blockIdx.x;
+    const uint64_t a = qoff[q], b = qoff[q + 1];
+    const bool on = sel == nullptr || sel[q] == AB_PATH_SPARSE;
+    for (uint64_t j = a + threadIdx.x; j < b; j += 256) {
+        const uint32_t k = keys[j];
+        uint64_t c = 0;
+        if (on && k < ix.nkeys) c = ix.seg8[(uint64_t)(k + 1) * 8 + 6] - ix.seg8[(uint64_t)k * 8 + 6];
+        cnt[j] = c;
+        kq[j] = q;
+    }
+}
+
+void launch_sparse_plan(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff, const uint8_t *sel,
+                        uint32_t nq, uint32_t *kq, uint64_t *cnt, hipStream_t s)
+{
+    if (!nq) return;
+    hipLaunchKernelGGL(k_sparse_plan, dim3(nq), dim3(256), 0, s, ix, keys, qoff, sel, kq, cnt);
+}
+
+// Largest j in [0, K) with koff[j] <= x (koff non-decreasing, koff[0] <= x <
+// koff[K]); (NT+1)-ary search by the whole work-group.
+template <int NT>
+__device__ uint64_t wg_find(const uint64_t *__restrict__ koff, uint64_t K, uint64_t x, unsigned long long *s_lo,
+                            unsigned long long *s_hi)
+{
+    uint64_t lo = 0, hi = K;
+    while (hi - lo > 1) {
+        if (threadIdx.x == 0) {
+            *s_lo = lo;
+            *s_hi = hi;
+        }
+        __syncthreads();
+        const uint64_t span = hi - lo;
+        const uint64_t p = lo + (span * (threadIdx.x + 1)) / (NT + 1);
+        if (p > lo && p < hi) {
+            if (koff[p] <= x)
+                atomicMax(s_lo, (unsigned long long)p);
+            else
+                atomicMin(s_hi, (unsigned long long)p);
+        }
+        __syncthreads();
+        lo = *s_lo;
+        hi = *s_hi;
+        __syncthreads();
+    }
+    return lo;
+}
+
+template <int T>
+__device__ __forceinline__ void emit_global(const DevIndex &ix, uint32_t idx, uint32_t *__restrict__ bm)
+{
+    const uint32_t h = ix.hdr[T][idx];
+    uint32_t w[8];
+    load_payload<T>(ix.pay[T], idx, w);
+    emit_record<T>(h, w, [&](uint32_t word, uint32_t bits) { atomicOr(bm + word, bits); });
+}
+
+// Work item = one record of one (query, key).  Items are numbered through the
+// prefix koff over keys; each work-group takes CH consecutive items (merge-path
+// split of the total over all lists, instead of the reference's CPU chunking
+// into <=256-record units, src/AdaBool/load_balance_transformer.h:50-69).
+constexpr uint32_t kSparseNT = 256, kSparseWin = 1024;
+__global__ __launch_bounds__(kSparseNT) void k_sparse_union(DevIndex ix, const uint32_t *__restrict__ keys,
+                                                            const uint32_t *__restrict__ kq,
+                                                            const uint64_t *__restrict__ koff, uint64_t K,
+                                                            uint32_t *__restrict__ out, uint64_t out_stride,
+                                                            uint32_t CH)
+{
+    __shared__ unsigned long long skoff[kSparseWin + 1];
+    __shared__ unsigned long long s_lo, s_hi;
+    if (K == 0) return;
+    const uint64_t W = koff[K];
+    const uint64_t nch = (W + CH - 1) / CH;
+    for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint64_t c0 = c * CH, c1 = (c0 + CH < W) ? c0 + CH : W;
+        uint64_t jw = wg_find<kSparseNT>(koff, K, c0, &s_lo, &s_hi);
+        for (;;) {
+            const uint64_t jn = (jw + kSparseWin < K) ? jw + kSparseWin : K;
+            const uint32_t nwin = (uint32_t)(jn - jw);
+            for (uint32_t i = threadIdx.x; i <= nwin; i += kSparseNT) skoff[i] = koff[jw + i];
+            __syncthreads();
+            const uint64_t wlo = skoff[0] > c0 ? skoff[0] : c0;
+            const uint64_t whi = skoff[nwin] < c1 ? skoff[nwin] : c1;
+            for (uint64_t it = wlo + threadIdx.x; it < whi; it += kSparseNT) {
+                uint32_t lo = 0, hi = nwin;  // skoff[lo] <= it < skoff[hi]
+                while (hi - lo > 1) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (skoff[mid] <= it) lo = mid; else hi = mid;
+                }
+                const uint64_t j = jw + lo;
+                uint32_t r = (uint32_t)(it - skoff[lo]);
+                const uint32_t k = keys[j];
+                const uint32_t q = kq[j];
+                const uint4 *rows = reinterpret_cast<const uint4 *>(ix.seg8 + (uint64_t)k * 8);
+                const uint4 a0 = rows[0], a1 = rows[1], b0 = rows[2], b1 = rows[3];
+                uint32_t *bm = out + (uint64_t)q * out_stride;
+                const uint32_t l0 = b0.x - a0.x, l1 = b0.y - a0.y, l2 = b0.z - a0.z, l3 = b0.w - a0.w,
+                               l4 = b1.x - a1.x;
+                if (r < l0) { emit_global<AB_ARRAY1>(ix, a0.x + r, bm); continue; }
+                r -= l0;
+                if (r < l1) { emit_global<AB_ARRAY4>(ix, a0.y + r, bm); continue; }
+                r -= l1;
+                if (r < l2) { emit_global<AB_ARRAY8>(ix, a0.z + r, bm); continue; }
+                r -= l2;
+                if (r < l3) { emit_global<AB_ARRAY16>(ix, a0.w + r, bm); continue; }
+                r -= l3;
+                if (r < l4) { emit_global<AB_ARRAY32>(ix, a1.x + r, bm); continue; }
+                r -= l4;
+                emit_global<AB_BITSET>(ix, a1.y + r, bm);
+            }
+            const bool done = (skoff[nwin] >= c1) || jn >= K;
+            __syncthreads();
+            if (done) break;
+            jw = jn;
+        }
+    }
+}
+
+void launch_sparse_union(const DevIndex &ix, const uint32_t *keys, const uint32_t *kq, const uint64_t *koff,
+                         uint64_t K, uint32_t *out, uint64_t out_stride, uint32_t chunk, uint32_t grid,
+                         hipStream_t s)
+{
+    if (!K || !grid) return;
+    hipLaunchKernelGGL(k_sparse_union, dim3(grid), dim3(kSparseNT), 0, s, ix, keys, kq, koff, K, out, out_stride,
+                       chunk);
+}
+
+// ---------------------------------------------------------------------------
+// dense path: query masks
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mask_keys(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ qoff,
+                                                   DenseGroup g, void *mask, uint32_t nkeys, int clear)
+{
+    const uint32_t b = blockIdx.y;
+    const uint32_t q = g.q[b];
+    const uint64_t a = qoff[q], e = qoff[q + 1];
+    for (uint64_t j = a + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < e; j += (uint64_t)gridDim.x * 256) {
+        const uint32_t k = keys[j];
+        if (k >= nkeys) continue;
+        if constexpr (MODE == kModeBitmap) {
+            uint32_t *m = reinterpret_cast<uint32_t *>(mask) + (k >> 5);
+            if (clear) *m = 0; else atomicOr(m, 1u << (k & 31));
+        } else if constexpr (MODE == kModeMask32) {
+            uint32_t *m = reinterpret_cast<uint32_t *>(mask) + k;
+            if (clear) *m = 0; else atomicOr(m, 1u << b);
+        } else {
+            unsigned long long *m = reinterpret_cast<unsigned long long *>(mask) + k;
+            if (clear) *m = 0; else atomicOr(m, 1ull << b);
+        }
+    }
+}
+
+void launch_mask_keys(int mode, const uint32_t *keys, const uint64_t *qoff, const DenseGroup &g, void *mask,
+                      uint32_t nkeys, int clear, hipStream_t s)
+{
+    dim3 grid(8, g.B);
+    if (mode == kModeBitmap)
+        hipLaunchKernelGGL(k_mask_keys<kModeBitmap>, grid, dim3(256), 0, s, keys, qoff, g, mask, nkeys, clear);
+    else if (mode == kModeMask32)
+        hipLaunchKernelGGL(k_mask_keys<kModeMask32>, grid, dim3(256), 0, s, keys, qoff, g, mask, nkeys, clear);
+    else
+        hipLaunchKernelGGL(k_mask_keys<kModeMask64>, grid, dim3(256), 0, s, keys, qoff, g, mask, nkeys, clear);
+}
+
+// mask[k] bit b = bit k of bitmap (b0 + b), for k < nkeys (every entry written)
+__global__ void k_mask_from_bitmaps(const uint32_t *__restrict__ bms, uint64_t len_u32, uint32_t b0, uint32_t B,
+                                    unsigned long long *mask, uint32_t nkeys)
+{
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nkeys; k += (uint64_t)gridDim.x * blockDim.x) {
+        unsigned long long m = 0;
+        const uint64_t w = k >> 5;
+        if (w < len_u32)
+            for (uint32_t b = 0; b < B; ++b) m |= (unsigned long long)((bms[(uint64_t)(b0 + b) * len_u32 + w] >> (k & 31)) & 1u) << b;
+        mask[k] = m;
+    }
+}
+
+void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_t b0, uint32_t B, void *mask,
+                              uint32_t nkeys, hipStream_t s)
+{
+    uint64_t grid = ((uint64_t)nkeys + 255) / 256;
+    if (grid > 8192) grid = 8192;
+    if (!grid) return;
+    hipLaunchKernelGGL(k_mask_from_bitmaps, dim3((uint32_t)grid), dim3(256), 0, s, bitmaps, len_u32, b0, B,
+                       (unsigned long long *)mask, nkeys);
+}
+
+// ---------------------------------------------------------------------------
+// dense path: value-tiled bitmap probe, one work-group per value tile
+// ---------------------------------------------------------------------------
+template <int MODE> struct MaskOf { using T = uint32_t; };
+template <> struct MaskOf<kModeMask64> { using T = unsigned long long; };
+
+template <int MODE>
+__device__ __forceinline__ typename MaskOf<MODE>::T probe(const void *__restrict__ m, uint32_t k)
+{
+    if constexpr (MODE == kModeBitmap)
+        return (reinterpret_cast<const uint32_t *>(m)[k >> 5] >> (k & 31)) & 1u;
+    else if constexpr (MODE == kModeMask32)
+        return reinterpret_cast<const uint32_t *>(m)[k];
+    else
+        return reinterpret_cast<const unsigned long long *>(m)[k];
+}
+
+template <int MODE, int T>
+__device__ __forceinline__ void dense_emit_block(const DevIndex &ix, uint32_t rec_base,
+                                                 const typename MaskOf<MODE>::T (&mk)[16], uint32_t lane,
+                                                 uint32_t *tiles, uint32_t tw, uint32_t tile_word0)
+{
+    using M = typename MaskOf<MODE>::T;
+    const uint32_t *__restrict__ hdr = ix.vhdr[T];
+    const uint8_t *__restrict__ pay = ix.vpay[T];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const M m = mk[s];
+        if (m) {
+            const uint32_t idx = rec_base + (uint32_t)s * 64 + lane;
+            const uint32_t h = hdr[idx];
+            uint32_t w[8];
+            load_payload<T>(pay, idx, w);
+            emit_record<T>(h, w, [&](uint32_t gw, uint32_t bits) {
+                const uint32_t lw = gw - tile_word0;
+                if constexpr (MODE == kModeBitmap) {
+                    atomicOr(&tiles[lw], bits);
+                } else {
+                    M mm = m;
+                    while (mm) {
+                        const uint32_t b = (uint32_t)__builtin_ctzll((unsigned long long)mm);
+                        mm &= mm - 1;
+                        atomicOr(&tiles[b * tw + lw], bits);
+                    }
+                }
+            });
+        }
+    }
+}
+
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__restrict__ mask, DenseGroup g,
+                                                    uint32_t *__restrict__ out, uint64_t out_stride,
+                                                    uint64_t out_words)
+{
+    using M = typename MaskOf<MODE>::T;
+    extern __shared__ uint32_t tiles[];
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tw = 1u << (ix.tile_shift - 5);
+    const uint32_t tile_word0 = tile * tw;
+    const uint32_t ntw = g.B * tw;
+    for (uint32_t i = threadIdx.x; i < ntw; i += NT) tiles[i] = 0;
+    __syncthreads();
+
+    const uint32_t lane = lane_id();
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    constexpr uint32_t NW = NT / 64;
+    const uint32_t b0 = ix.tile_blk[tile], b1 = ix.tile_blk[tile + 1];
+    for (uint32_t b = b0 + wave; b < b1; b += NW) {
+        const uint4 mt = ix.vblk[b];
+        const uint32_t type = mt.z & 15u, wl = (mt.z >> 4) & 3u, nv = mt.z >> 8;
+        const uint8_t *kp = ix.vkeys + mt.y;
+        uint32_t key[16];
+        if (wl == 0) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(kp + lane * 16u);
+            const uint32_t ww[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int s = 0; s < 16; ++s) key[s] = (ww[s >> 2] >> (8 * (s & 3))) & 0xffu;
+        } else if (wl == 1) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(kp + lane * 32u);
+            const uint4 v0 = p[0], v1 = p[1];
+            const uint32_t ww[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+            for (int s = 0; s < 16; ++s) key[s] = (ww[s >> 1] >> (16 * (s & 1))) & 0xffffu;
+        } else {
+            const uint4 *p = reinterpret_cast<const uint4 *>(kp + lane * 64u);
+            const uint4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
+            const uint32_t ww[16] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w,
+                                     v2.x, v2.y, v2.z, v2.w, v3.x, v3.y, v3.z, v3.w};
+#pragma unroll
+            for (int s = 0; s < 16; ++s) key[s] = ww[s];
+        }
+        M mk[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const uint32_t r = (uint32_t)s * 64 + lane;
+            mk[s] = r < nv ? probe<MODE>(mask, mt.x + key[s]) : (M)0;
+        }
+        switch (type) {
+        case AB_ARRAY1: dense_emit_block<MODE, AB_ARRAY1>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY4: dense_emit_block<MODE, AB_ARRAY4>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY8: dense_emit_block<MODE, AB_ARRAY8>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY16: dense_emit_block<MODE, AB_ARRAY16>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY32: dense_emit_block<MODE, AB_ARRAY32>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
+        default: dense_emit_block<MODE, AB_BITSET>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
+        }
+    }
+    __syncthreads();
+    // write-once: every word of the tile, for every query of the group
+    const uint64_t nwq = out_words > tile_word0 ? ((out_words - tile_word0) < tw ? out_words - tile_word0 : tw) : 0;
+    for (uint32_t b = 0; b < g.B; ++b) {
+        uint32_t *dst = out + (uint64_t)g.q[b] * out_stride + tile_word0;
+        for (uint32_t i = threadIdx.x; i < nwq; i += NT) dst[i] = tiles[b * tw + i];
+    }
+}
+
+uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift) { return B * (1u << (tile_shift - 5)) * 4u; }
+
+template <int MODE>
+static hipError_t launch_dense_mode(uint32_t threads, const DevIndex &ix, const void *mask, const DenseGroup &g,
+                                    uint32_t *out, uint64_t out_stride, uint64_t out_words, hipStream_t s)
+{
+    const uint32_t lds = dense_lds_bytes(g.B, ix.tile_shift);
+    dim3 grid(ix.ntiles);
+    switch (threads) {
+    case 64: hipLaunchKernelGGL((k_dense_tiles<MODE, 64>), grid, dim3(64), lds, s, ix, mask, g, out, out_stride, out_words); break;
+    case 128: hipLaunchKernelGGL((k_dense_tiles<MODE, 128>), grid, dim3(128), lds, s, ix, mask, g, out, out_stride, out_words); break;
+    case 512: hipLaunchKernelGGL((k_dense_tiles<MODE, 512>), grid, dim3(512), lds, s, ix, mask, g, out, out_stride, out_words); break;
+    case 1024: hipLaunchKernelGGL((k_dense_tiles<MODE, 1024>), grid, dim3(1024), lds, s, ix, mask, g, out, out_stride, out_words); break;
+    default: hipLaunchKernelGGL((k_dense_tiles<MODE, 256>), grid, dim3(256), lds, s, ix, mask, g, out, out_stride, out_words); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask, const DenseGroup &g,
+                              uint32_t *out, uint64_t out_stride, uint64_t out_words, hipStream_t s)
+{
+    if (!ix.ntiles || !g.B) return hipSuccess;
+    if (mode == kModeBitmap) return launch_dense_mode<kModeBitmap>(threads, ix, mask, g, out, out_stride, out_words, s);
+    if (mode == kModeMask32) return launch_dense_mode<kModeMask32>(threads, ix, mask, g, out, out_stride, out_words, s);
+    return launch_dense_mode<kModeMask64>(threads, ix, mask, g, out, out_stride, out_words, s);
+}
+
+// ---------------------------------------------------------------------------
+// bitmap -> ascending ids (CollectIdsFromBitmap, test/adsfilter_q_all.cu:108-119)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_popcount(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
+                                                  uint32_t nchunk, uint32_t *ccnt)
+{
+    __shared__ uint32_t sh[4];
+    const uint32_t q = blockIdx.y, c = blockIdx.x;
+    const uint64_t w0 = (uint64_t)c * kCompactWords;
+    const uint64_t w1 = w0 + kCompactWords < words ? w0 + kCompactWords : words;
+    const uint32_t *src = bm + (uint64_t)q * stride;
+    uint32_t s = 0;
+    for (uint64_t i = w0 + threadIdx.x; i < w1; i += 256) s += __popc(src[i]);
+    s = wave_sum(s);
+    if (lane_id() == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) ccnt[(uint64_t)q * nchunk + c] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, uint32_t *ccnt, hipStream_t s)
+{
+    if (!nq || !words) return;
+    uint32_t nchunk = (uint32_t)((words + kCompactWords - 1) / kCompactWords);
+    hipLaunchKernelGGL(k_popcount, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt);
+}
+
+// Each wave expands 512 consecutive words: per 64-word step, a wave prefix of
+// the words' popcounts places every lane's ids in an LDS staging area that the
+// wave then stores to the id list with coalesced writes.
+__global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
+                                                   uint32_t nchunk, const uint32_t *__restrict__ ccnt,
+                                                   uint32_t *__restrict__ ids, uint64_t ids_stride,
+                                                   uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t stage[4][64 * 32];
+    __shared__ uint32_t wcnt[4];
+    __shared__ uint32_t sred[4];
+    const uint32_t q = blockIdx.y, c = blockIdx.x;
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t *cc = ccnt + (uint64_t)q * nchunk;
+    uint32_t pre = 0;
+    for (uint32_t i = threadIdx.x; i < c; i += 256) pre += cc[i];
+    pre = wave_sum(pre);
+    if (lane == 0) sred[wave] = pre;
+    __syncthreads();
+    const uint64_t prefix = (uint64_t)sred[0] + sred[1] + sred[2] + sred[3];
+    if (c == nchunk - 1 && threadIdx.x == 0 && counts) counts[q] = (uint32_t)(prefix + cc[c]);
+    if (!ids) return;
+    const uint32_t *src = bm + (uint64_t)q * stride;
+    const uint64_t ww0 = (uint64_t)c * kCompactWords + (uint64_t)wave * 512;
+    uint32_t mine = 0;
+    for (uint32_t i = 0; i < 8; ++i) {
+        const uint64_t wi = ww0 + i * 64 + lane;
+        if (wi < words) mine += __popc(src[wi]);
+    }
+    mine = wave_sum(mine);
+    if (lane == 0) wcnt[wave] = mine;
+    __syncthreads();
+    uint64_t off = prefix;
+    for (uint32_t w = 0; w < wave; ++w) off += wcnt[w];
+    uint32_t *dst = ids + (uint64_t)q * ids_stride;
+    uint32_t *stg = stage[wave];
+    for (uint32_t i = 0; i < 8; ++i) {
+        const uint64_t wi = ww0 + i * 64 + lane;
+        uint32_t x = wi < words ? src[wi] : 0u;
+        const uint32_t cnt = __popc(x);
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t tot = __shfl(incl, 63, 64);
+        if (tot == 0) continue;
+        uint32_t p = incl - cnt;
+        const uint32_t base = (uint32_t)(wi * 32);
+        while (x) {
+            const uint32_t b = __builtin_ctz(x);
+            stg[p++] = base + b;
+            x &= x - 1;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (uint32_t j = lane; j < tot; j += 64) dst[off + j] = stg[j];
+        off += tot;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
+void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, const uint32_t *ccnt,
+                      uint32_t *ids, uint64_t ids_stride, uint32_t *counts, hipStream_t s)
+{
+    if (!nq || !words) return;
+    uint32_t nchunk = (uint32_t)((words + kCompactWords - 1) / kCompactWords);
+    hipLaunchKernelGGL(k_write_ids, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt, ids,
+                       ids_stride, counts);
+}
+
+// ---------------------------------------------------------------------------
+// cost-model features and the threshold rule
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_features(DevIndex ix, const uint32_t *__restrict__ keys,
+                                                 const uint64_t *__restrict__ qoff, uint64_t R, float *feats,
+                                                 uint64_t *Mout)
+{
+    constexpr uint32_t kBytes[6] = {4, 8, 12, 20, 36, 36};
+    const uint32_t q = blockIdx.x, lane = lane_id();
+    const uint64_t a = qoff[q], e = qoff[q + 1];
+    unsigned long long valid = 0, M = 0, mx = 0, bytes = 0, nv = 0, a1 = 0, bs = 0;
+    double sq = 0;
+    for (uint64_t j = a + lane; j < e; j += 64) {
+        const uint32_t k = keys[j];
+        if (k >= ix.nkeys) continue;
+        const uint32_t *r0 = ix.seg8 + (uint64_t)k * 8, *r1 = r0 + 8;
+        const unsigned long long tot = r1[6] - r0[6];
+        ++valid;
+        M += tot;
+        sq += (double)tot * (double)tot;
+        mx = tot > mx ? tot : mx;
+        for (int t = 0; t < 6; ++t) bytes += (unsigned long long)(r1[t] - r0[t]) * kBytes[t];
+        nv += r1[7] - r0[7];
+        a1 += r1[0] - r0[0];
+        bs += r1[5] - r0[5];
+    }
+    valid = wave_sum64(valid);
+    M = wave_sum64(M);
+    bytes = wave_sum64(bytes);
+    nv = wave_sum64(nv);
+    a1 = wave_sum64(a1);
+    bs = wave_sum64(bs);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        sq += __shfl_xor(sq, d, 64);
+        unsigned long long o = __shfl_xor(mx, d, 64);
+        mx = o > mx ? o : mx;
+    }
+    if (lane == 0) {
+        const double n = (double)(e - a);
+        const double dv = valid ? (double)valid : 1.0;
+        const double mean = (double)M / dv;
+        double var = sq / dv - mean * mean;
+        if (var < 0) var = 0;
+        const double dM = M ? (double)M : 1.0;
+        const double nvd = ix.nvalues ? (double)ix.nvalues : 1.0;
+        float *f = feats + (uint64_t)q * AB_NFEATURES;
+        f[0] = (float)log1p(n);
+        f[1] = (float)log1p((double)valid);
+        f[2] = (float)log1p((double)M);
+        f[3] = (float)log1p((double)mx);
+        f[4] = (float)log1p(sqrt(var));
+        f[5] = (float)log1p((double)bytes);
+        f[6] = (float)((double)M / (R ? (double)R : 1.0));
+        const double hd = (double)nv / nvd;
+        f[7] = (float)(hd < 1.0 ? hd : 1.0);
+        f[8] = (float)(1.0 - exp(-hd));
+        f[9] = (float)((double)valid / (ix.nkeys ? (double)ix.nkeys : 1.0));
+        f[10] = (float)((double)a1 / dM);
+        f[11] = (float)((double)bs / dM);
+        if (Mout) Mout[q] = M;
+    }
+}
+
+void launch_features(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff, uint32_t nq, uint64_t R,
+                     float *feats, uint64_t *M, hipStream_t s)
+{
+    if (!nq) return;
+    hipLaunchKernelGGL(k_features, dim3(nq), dim3(64), 0, s, ix, keys, qoff, R, feats, M);
+}
+
+__global__ void k_decide_threshold(const uint64_t *M, uint32_t nq, double thr, uint8_t *sel)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nq) sel[q] = (double)M[q] >= thr ? (uint8_t)AB_PATH_DENSE : (uint8_t)AB_PATH_SPARSE;
+}
+
+void launch_decide_threshold(const uint64_t *M, uint32_t nq, double thr, uint8_t *sel, hipStream_t s)
+{
+    if (!nq) return;
+    hipLaunchKernelGGL(k_decide_threshold, dim3((nq + 255) / 256), dim3(256), 0, s, M, nq, thr, sel);
+}
+
+}  // namespace ab

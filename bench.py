@@ -1,0 +1,257 @@
+#!/usr/bin/env python3
+"""Benchmark of the AdaBool OR-filtering path on one MI355X.
+
+A step is one batched pass over the 1024 C2 queries (4M values x 1M keys,
+~80M pairs, Zipf s=1.1; LogNormal(ln 200, 1.2) query sizes in [8, 20000] with
+Zipf-weighted keys): per-query dispatch (learned cost model when one is
+loaded, else the threshold rule), the persistent sparse union and the grouped
+dense tile kernels, then bitmap -> ascending-id compaction.  Inputs (index and
+query keys) are resident in HBM before timing starts; every step writes every
+query's full result bitmap, id list and count.
+
+Prints one JSON line; see the metric definition in the output.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def list_bytes_per_query(ix, keys: np.ndarray, qoff: np.ndarray):
+    """Compressed-list bytes (4 + payload per record) each query's keys own."""
+    import adabool_amd as ab
+
+    nk = ix.info["nkeys"]
+    valid = keys < nk
+    kk = np.where(valid, keys, 0).astype(np.int64)
+    per_key = np.zeros(keys.size, np.float64)
+    for t in range(ab.NTYPES):
+        segs = _segs(ix, t)
+        cnt = (segs[kk + 1].astype(np.int64) - segs[kk].astype(np.int64)) * valid
+        per_key += cnt * (4 + ab.PAYLOAD_BYTES[t])
+    cs = np.concatenate([[0.0], np.cumsum(per_key)])
+    return cs[qoff[1:].astype(np.int64)] - cs[qoff[:-1].astype(np.int64)]
+
+
+_SEG_CACHE = {}
+
+
+def _segs(ix, t):
+    import ctypes as C
+
+    import adabool_amd as ab
+
+    key = (id(ix), t)
+    if key not in _SEG_CACHE:
+        segs = np.zeros(ix.info["nkeys"] + 1, np.uint32)
+        ab._check(ab._lib.ab_index_export_type(ix._h, t, None, None, segs.ctypes.data))
+        _SEG_CACHE[key] = segs
+    return _SEG_CACHE[key]
+
+
+def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float):
+    """Oracle (plain C, CSR-by-key union -> bitmap + ids) on the host cores,
+    over a bounded prefix of the same queries; also checks the GPU counts of
+    that prefix against it."""
+    import oracle
+
+    nthreads = min(16, os.cpu_count() or 1)
+    csr = oracle.CSR(pairs_k, pairs_v, spec.nkeys, spec.nvalues)
+    nq = qoff.size - 1
+    # calibrate on a few queries, then size the sample to ~budget_s of work
+    n0 = min(nq, 2 * nthreads)
+    t0, c0, _, _ = csr.union_batch(keys, qoff[: n0 + 1], nthreads)
+    per_q = max(t0 / n0, 1e-6)
+    ns = int(min(nq, max(n0, budget_s / per_q)))
+    sub = qoff[: ns + 1]
+    t, counts, lat, _ = csr.union_batch(keys[: int(sub[-1])], sub, nthreads)
+    ok = bool(np.array_equal(counts.astype(np.int64), counts_gpu[:ns].astype(np.int64)))
+    return {"value": ns / t, "unit": "queries/sec", "cores": nthreads, "kind": "port",
+            "sample": f"first {ns} of the {nq} C2 queries, oracle/oracle.c or_union_batch, {nthreads} threads, "
+                      f"{t:.1f} s", "counts_match_gpu": ok,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--scale", type=float, default=1.0, help="filter size factor (1.0 = C2)")
+    ap.add_argument("--path", default="auto", choices=["auto", "threshold", "dense", "sparse", "learned"])
+    ap.add_argument("--baseline-queries", type=int, default=256)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the path shards by value window across GPUs; "
+              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import adabool_amd as ab
+    from adabool_amd import configs
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    spec = configs.C2 if args.scale == 1.0 else configs.scaled(configs.C2, args.scale)
+    t = time.time()
+    pk, pv = configs.make_filter(spec)
+    log(f"filter {spec.name}: {pk.size} pairs in {time.time() - t:.1f}s")
+    t = time.time()
+    ix = ab.Index(pk, pv, spec.nkeys, spec.nvalues)
+    info = ix.info
+    log(f"index: R={info['nrec_total']} list_bytes={info['list_bytes']/1e6:.1f}MB key_stream="
+        f"{info['key_stream_bytes']/1e6:.1f}MB tiles={info['ntiles']} device={info['device_bytes']/1e6:.0f}MB "
+        f"in {time.time() - t:.1f}s")
+    qk, qo = configs.make_queries(spec, configs.C2_Q, args.queries)
+    nq = qo.size - 1
+    ctx = ab.Context(ix)
+    path = {"auto": ab.AB_PATH_AUTO, "threshold": ab.AB_PATH_THRESHOLD, "dense": ab.AB_PATH_DENSE,
+            "sparse": ab.AB_PATH_SPARSE, "learned": ab.AB_PATH_LEARNED}[args.path]
+    dispatch = "threshold"
+    if path in (ab.AB_PATH_AUTO, ab.AB_PATH_LEARNED) and hasattr(ctx, "load_model") and os.path.exists(args.model):
+        ctx.load_model(args.model)
+        dispatch = "learned"
+
+    sw = ix.stride_words
+    nv = spec.nvalues
+    d_keys = torch.from_numpy(qk.view(np.int32)).cuda()
+    d_bm = torch.empty(nq * sw, dtype=torch.int32, device="cuda")
+    d_ids = torch.empty(nq * nv, dtype=torch.int32, device="cuda")
+    d_cnt = torch.empty(nq, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        return ctx.query_batch_device(d_keys, qo, path, d_bm, d_ids, nv, d_cnt)
+
+    for _ in range(args.warmup):
+        paths = step()
+    ctx.sync()
+    ctx.timing(reset=True)
+    ctx.set_timing(True)
+    step_ms = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        paths = step()
+        ctx.sync()
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = ctx.timing(reset=True)
+    ctx.set_timing(False)
+    counts = d_cnt.cpu().numpy().view(np.uint32).astype(np.int64)
+    qps = nq * args.steps / wall
+    ndense = int((paths == ab.AB_PATH_DENSE).sum())
+    log(f"batched: {qps:.0f} q/s, {wall / args.steps * 1e3:.2f} ms/step, dense {ndense}/{nq}")
+    log("kernels: " + ", ".join(f"{k}={v[0] / args.steps:.3f}ms/{v[1] // max(args.steps, 1)}" for k, v in kt.items()))
+
+    # ---- algorithmic bytes (per query: 4|q| + list bytes + bitmap + 4|R| + 4)
+    lb = list_bytes_per_query(ix, qk, qo)
+    nkeys_q = np.diff(qo).astype(np.float64)
+    stride_b = float(ix.stride_bytes)
+    alg_q = 4 * nkeys_q + lb + stride_b + 4 * counts + 4
+    job_gbs = alg_q.sum() * args.steps / wall / 1e9
+    # per kernel class: bytes per step of that class
+    dense_mask = paths == ab.AB_PATH_DENSE
+    kbytes = {
+        "write_ids": float((stride_b + 4 * counts + 4).sum()),
+        "dense_tiles": float((lb[dense_mask] + stride_b).sum()),
+        "sparse_union": float((4 * nkeys_q[~dense_mask] + lb[~dense_mask]).sum()),
+        "popcount": float(stride_b * nq),
+        "sparse_zero": float(stride_b * (~dense_mask).sum()),
+    }
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms_total, launches = kt[dom]
+    roof = None
+    if dom in kbytes and launches:
+        per_launch_bytes = kbytes[dom] * args.steps / launches
+        avg_ms = ms_total / launches
+        ach = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms, "launches": launches}
+
+    # ---- plain per-query threshold dispatch (one query at a time, synchronized)
+    nb = min(args.baseline_queries, nq)
+    d_bm1 = torch.empty(sw, dtype=torch.int32, device="cuda")
+    d_ids1 = torch.empty(nv, dtype=torch.int32, device="cuda")
+    d_c1 = torch.empty(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    for i in range(min(8, nb)):
+        ctx.query_device(qk[qo[i]:qo[i + 1]], ab.AB_PATH_THRESHOLD, d_bm1, d_ids1, d_c1)
+        ctx.sync()
+    lat1 = []
+    mism = 0
+    t1 = time.perf_counter()
+    for i in range(nb):
+        s0 = time.perf_counter()
+        ctx.query_device(qk[qo[i]:qo[i + 1]], ab.AB_PATH_THRESHOLD, d_bm1, d_ids1, d_c1)
+        ctx.sync()
+        lat1.append((time.perf_counter() - s0) * 1e3)
+        if i < 16:
+            mism += int(int(d_c1.cpu().numpy().view(np.uint32)[0]) != counts[i])
+    base_qps = nb / (time.perf_counter() - t1)
+    log(f"per-query threshold: {base_qps:.0f} q/s, p50 {np.percentile(lat1, 50):.3f} ms "
+        f"p99 {np.percentile(lat1, 99):.3f} ms; count mismatches vs batch: {mism}")
+
+    cpu = None
+    if not args.no_cpu:
+        t = time.time()
+        cpu = cpu_baseline(pk, pv, spec, qk, qo, counts, args.cpu_budget)
+        log(f"cpu baseline: {cpu['value']:.1f} q/s on {cpu['cores']} threads ({time.time() - t:.1f}s), "
+            f"counts match: {cpu['counts_match_gpu']}")
+
+    out = {
+        "metric": "queries/sec + p99 latency, learned vs threshold dispatch, bit-exact",
+        "value": round(qps, 1),
+        "unit": "queries/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded generators, adabool_amd/configs.py)",
+        "config": {"workload": f"{spec.name}: {spec.nvalues} values x {spec.nkeys} keys, {pk.size} pairs, "
+                               f"Zipf {spec.zipf_s}; {nq} Zipf-weighted queries, LogNormal(ln 200, 1.2) sizes "
+                               f"in [8, 20000]; bitmap + ascending ids + counts",
+                   "queries_per_step": nq, "dispatch": dispatch, "dense_queries": ndense,
+                   "tile_shift": info["tile_shift"]},
+        "latency_ms": {"batch_p50": float(np.percentile(step_ms, 50)), "batch_p99": float(np.percentile(step_ms, 99)),
+                       "per_query_threshold_p50": float(np.percentile(lat1, 50)),
+                       "per_query_threshold_p99": float(np.percentile(lat1, 99))},
+        "per_query_threshold_qps": round(base_qps, 1),
+        "speedup_vs_per_query_threshold": round(qps / base_qps, 2),
+        "job_algorithmic_GBps": round(job_gbs, 1),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,280 @@
+/*
+ * adabool.h -- C ABI of the MI355X-native AdaBool OR-filtering engine.
+ *
+ * Given a sparse Boolean matrix of uint32 (key, value) pairs -- key = term,
+ * value = row id -- and a query set of keys, the engine returns the bitmap and
+ * the ascending id list of every value that some query key maps to:
+ *     R(q) = { v | (k, v) in filter, k in q, k < nkeys }.
+ *
+ * Conventions
+ *   - Every function returns ab_status; ab_last_error() gives a thread-local
+ *     message for the last failure.  No C++ exception crosses this boundary.
+ *   - Index handles own their device memory.  Contexts borrow the index, which
+ *     must outlive them.  Inputs are borrowed for the duration of the call.
+ *   - Pointers named d_* are device pointers (hipMalloc'd memory of the
+ *     current device), h_* are host pointers.
+ *   - Result bitmaps: bit v of a query's bitmap is bit (v & 31) of uint32
+ *     word v >> 5 (equivalently bit v & 7 of byte v >> 3).  One query's bitmap
+ *     spans ab_result_stride_bytes() = align_up(ceil(nvalues / 8), 16) bytes;
+ *     padding bits are zero.  Every call WRITES the full bitmap (it is never
+ *     OR-ed into caller memory), so callers need no memset.
+ *   - Ids are ascending uint32 values of R(q).
+ *   - Work is enqueued on the context's stream; call ab_ctx_sync() (or
+ *     synchronize the stream) before reading results.
+ *   - One context per host thread; an index may be shared read-only by many
+ *     contexts.
+ */
+#ifndef ADABOOL_H
+#define ADABOOL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ab_status {
+    AB_OK = 0,
+    AB_INVALID_ARG = 1,
+    AB_OOM = 2,
+    AB_DEVICE_ERROR = 3,
+    AB_UNSUPPORTED = 4,
+    AB_IO_ERROR = 5
+} ab_status;
+
+/* Which method evaluates a query. */
+typedef enum ab_path {
+    AB_PATH_AUTO = 0,      /* learned cost model if one is loaded, else threshold */
+    AB_PATH_DENSE = 1,     /* dense bitmap-probe over the value-tiled key stream */
+    AB_PATH_SPARSE = 2,    /* sparse load-balanced list union */
+    AB_PATH_THRESHOLD = 3, /* dense iff M(q) >= tau * R */
+    AB_PATH_LEARNED = 4    /* MLP cost model; falls back to threshold without one */
+} ab_path;
+
+/* Record types of the compressed lists (src/AdaBool/inverted_list.h:16-26). */
+enum {
+    AB_ARRAY1 = 0,
+    AB_ARRAY4 = 1,
+    AB_ARRAY8 = 2,
+    AB_ARRAY16 = 3,
+    AB_ARRAY32 = 4,
+    AB_BITSET = 5,
+    AB_NTYPES = 6
+};
+
+#define AB_NFEATURES 12
+
+typedef struct ab_index ab_index;
+typedef struct ab_ctx ab_ctx;
+
+typedef struct ab_index_info {
+    uint32_t nkeys;            /* key space: keys >= nkeys match nothing */
+    uint32_t nvalues;          /* value space: bits per result bitmap */
+    uint64_t nnz;              /* distinct (key, value) pairs */
+    uint64_t nrec[AB_NTYPES];  /* records per type */
+    uint64_t nrec_total;       /* R */
+    uint64_t list_bytes;       /* sum over records of 4 + payload bytes */
+    uint32_t tile_shift;       /* value tile = 2^tile_shift values */
+    uint32_t ntiles;
+    uint64_t nblocks;          /* key-stream blocks of the value-tiled layout */
+    uint64_t key_stream_bytes; /* delta bytes of the value-tiled key stream */
+    uint64_t device_bytes;     /* device memory held by the index */
+    uint64_t stride_bytes;     /* result bitmap bytes per query */
+} ab_index_info;
+
+/* ---- errors --------------------------------------------------------------- */
+const char *ab_last_error(void);
+const char *ab_status_str(int status);
+
+/* ---- index ---------------------------------------------------------------- */
+/* Builds the compressed index from nnz (key, value) pairs and uploads it.
+ * Replaces ivf_list<256,64>::BuildFrom (src/AdaBool/inverted_list.h:348-374).
+ * nkeys / nvalues: 0 derives max+1 from the pairs (test/adsfilter_q_all.cu:
+ * 359-360); a larger value widens the space; a smaller one is AB_INVALID_ARG.
+ * Duplicate pairs are removed.  tile_shift (8..16, 0 = auto) sets the value
+ * tile of the dense layout. */
+ab_status ab_index_build(const uint32_t *h_keys, const uint32_t *h_values, uint64_t nnz,
+                         uint32_t nkeys, uint32_t nvalues, uint32_t tile_shift,
+                         ab_index **out);
+
+/* ReadCOO + BuildFrom (test/adsfilter_q_all.cu:127-161, 340-357): text file of
+ * "key value" lines; blank lines and lines starting with '#' or '%' skipped. */
+ab_status ab_index_load_coo(const char *path, uint32_t tile_shift, ab_index **out);
+
+/* Same build without any device allocation: only info, export, decode and
+ * host feature calls accept such a handle.  For hosts without a GPU. */
+ab_status ab_index_build_host(const uint32_t *h_keys, const uint32_t *h_values, uint64_t nnz,
+                              uint32_t nkeys, uint32_t nvalues, uint32_t tile_shift,
+                              ab_index **out);
+
+ab_status ab_index_free(ab_index *ix);
+ab_status ab_index_get_info(const ab_index *ix, ab_index_info *out);
+uint64_t ab_result_stride_bytes(const ab_index *ix);
+
+/* Copies one record type's arrays back to the host (headers: nrec u32,
+ * payload: nrec * payload-bytes, segs: nkeys+1 u32 prefix counts per key, as
+ * key_segments_ in src/AdaBool/inverted_list.h:461-468).  NULL pointers skip. */
+ab_status ab_index_export_type(const ab_index *ix, int type, uint32_t *h_headers,
+                               uint8_t *h_payload, uint32_t *h_segs);
+
+/* Decodes every record of the value-tiled layout (the dense path's copy) back
+ * to (key, value) pairs, in layout order; returns the pair count (outputs may
+ * be NULL to count only, else cap pairs are written at most). */
+int64_t ab_index_decode_tiles(const ab_index *ix, uint32_t *h_keys, uint32_t *h_values,
+                              uint64_t cap);
+
+/* Key-block metadata of a sorted key array, as ComputeKeyBlockMeta
+ * (src/AdaBool/inverted_list.h:90-148).  Returns the padded buffer bytes; with
+ * non-NULL outputs fills starts[nblocks+1], pos[nblocks+1], buf. */
+uint64_t ab_key_block_meta(uint32_t step, uint64_t len, const uint32_t *keys,
+                           uint32_t *starts, uint32_t *pos, uint8_t *buf);
+
+/* ---- contexts ------------------------------------------------------------- */
+/* stream: a hipStream_t, or NULL for a stream owned by the context. */
+ab_status ab_ctx_create(ab_index *ix, void *stream, ab_ctx **out);
+ab_status ab_ctx_free(ab_ctx *ctx);
+ab_status ab_ctx_sync(ab_ctx *ctx);
+void *ab_ctx_stream(ab_ctx *ctx);
+/* Threshold rule: dense iff M(q) >= tau * R (M = records the query matches). */
+ab_status ab_ctx_set_threshold(ab_ctx *ctx, double tau);
+double ab_ctx_get_threshold(const ab_ctx *ctx);
+/* Dense group width (queries per key-stream pass, 1..64). */
+ab_status ab_ctx_set_dense_group(ab_ctx *ctx, uint32_t width);
+/* Work-group size of the dense kernel (64, 128, 256, 512 or 1024 threads). */
+ab_status ab_ctx_set_dense_threads(ab_ctx *ctx, uint32_t threads);
+/* Records per work item of the sparse kernel (power of two, 256..8192). */
+ab_status ab_ctx_set_sparse_chunk(ab_ctx *ctx, uint32_t items);
+
+/* Kernel classes reported by ab_ctx_timing. */
+enum {
+    AB_K_DECIDE = 0,      /* device features + dispatch decision */
+    AB_K_SPARSE_ZERO = 1, /* zero-fill of sparse queries' bitmaps */
+    AB_K_SPARSE_PLAN = 2, /* per-key extents + prefix scan */
+    AB_K_SPARSE_UNION = 3,
+    AB_K_DENSE_MASK = 4,  /* query mask set + clear */
+    AB_K_DENSE_TILES = 5,
+    AB_K_POPCOUNT = 6,
+    AB_K_WRITE_IDS = 7,
+    AB_K_NCLASSES = 8
+};
+/* Brackets every kernel class of later calls with HIP events on the context
+ * stream (off by default; on costs a few microseconds per call). */
+ab_status ab_ctx_set_timing(ab_ctx *ctx, int on);
+/* Synchronizes, then adds the elapsed ms and launch counts per class since
+ * the last reset into ms[AB_K_NCLASSES] and launches[AB_K_NCLASSES]
+ * (overwritten); reset != 0 clears the accumulators. */
+ab_status ab_ctx_timing(ab_ctx *ctx, double *ms, uint64_t *launches, int reset);
+
+/* ---- single query --------------------------------------------------------- */
+/* One query of n host keys (duplicates and keys >= nkeys allowed).  Writes
+ * the bitmap (d_bitmap, may be NULL for ids only), the ascending ids (d_ids,
+ * capacity nvalues, may be NULL) and the id count (d_count, device u32, may be
+ * NULL).  h_chosen (may be NULL) receives the path that ran.
+ * Replaces Prepare(keys) / Prepare(len, bitmap) + ListUnion_sparse /
+ * ListUnion_dense (src/AdaBool/inverted_list.h:481-543,
+ * test/adsfilter_q_all.cu:26-105) with the dispatch those callers lacked. */
+ab_status ab_query(ab_ctx *ctx, const uint32_t *h_keys, uint32_t n, int path,
+                   uint32_t *d_bitmap, uint32_t *d_ids, uint32_t *d_count, int *h_chosen);
+
+/* Sparse method on host keys into d_res (full stride written):
+ * Prepare(keys) + ListUnion_sparse (src/AdaBool/inverted_list.h:481-527,
+ * test/adsfilter_q_all.cu:64-105). */
+ab_status ab_list_union_sparse(ab_ctx *ctx, const uint32_t *h_keys, uint32_t n,
+                               uint32_t *d_res);
+
+/* Dense method on a host key bitmap of len_bytes bytes (bit k set = key k in
+ * the query): Prepare(len, bitmap) + ListUnion_dense
+ * (src/AdaBool/inverted_list.h:530-543, test/adsfilter_q_all.cu:26-61). */
+ab_status ab_list_union_dense(ab_ctx *ctx, uint64_t len_bytes, const uint8_t *h_bitmap,
+                              uint32_t *d_res);
+
+/* Batched dense method on B device bitmaps laid out query-major, each
+ * len_bytes bytes (a multiple of 4); query i's result goes to
+ * d_res + i * res_len_u32.  PrepareBatchDevice + RunArrays
+ * (src/AdaBool/inverted_list.h:591-605, src/AdaBool/inverted_list.cu.h:
+ * 224-228), for any B >= 1 (the reference handled 1..32). */
+ab_status ab_list_union_dense_batch(ab_ctx *ctx, uint32_t batch, uint64_t len_bytes,
+                                    const uint8_t *d_bitmaps, uint32_t *d_res,
+                                    uint32_t res_len_u32);
+
+/* Batched sparse method: n flattened host keys, h_qidx[i] = query of key i
+ * (0..nq-1); results at d_res + q * res_len_bytes / 4.  BatchListUnion_sparse
+ * (test/adsfilter_batch.cu:39-79, 473-484). */
+ab_status ab_list_union_sparse_batch(ab_ctx *ctx, const uint32_t *h_keys,
+                                     const uint32_t *h_qidx, uint32_t n, uint32_t nq,
+                                     uint32_t *d_res, uint32_t res_len_bytes);
+
+/* ---- batches -------------------------------------------------------------- */
+/* nq queries whose keys are concatenated in device memory d_keys, query i
+ * owning d_keys[h_qoff[i] .. h_qoff[i+1]).  path applies to every query
+ * (AUTO/THRESHOLD/LEARNED dispatch per query).  Outputs (each may be NULL):
+ *   d_bitmaps  nq * stride bytes (query i at i * stride),
+ *   d_ids      query i's ids at d_ids + i * ids_stride (ids_stride >= the
+ *              query's count; nvalues is always enough),
+ *   d_counts   nq u32 id counts,
+ *   h_paths    nq bytes: the path each query took (AB_PATH_DENSE/SPARSE).
+ * Queries of either path are packed into one union launch of the persistent
+ * kernel; the whole batch is replayed from a cached HIP graph when the same
+ * shape recurs. */
+ab_status ab_query_batch(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_qoff,
+                         uint32_t nq, int path, uint32_t *d_bitmaps, uint32_t *d_ids,
+                         uint64_t ids_stride, uint32_t *d_counts, uint8_t *h_paths);
+
+/* Bitmap -> ascending ids for nq bitmaps at d_bitmaps + i * stride
+ * (CollectIdsFromBitmap, test/adsfilter_q_all.cu:108-119, on the device). */
+ab_status ab_compact_ids(ab_ctx *ctx, const uint32_t *d_bitmaps, uint32_t nq,
+                         uint32_t *d_ids, uint64_t ids_stride, uint32_t *d_counts);
+
+/* ---- cost model ----------------------------------------------------------- */
+/* Per-query features (AB_NFEATURES floats per query) computed on the device
+ * for a device batch; h_features receives nq * AB_NFEATURES floats and h_m
+ * (may be NULL) the matched-record counts M(q). */
+ab_status ab_query_features(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_qoff,
+                            uint32_t nq, float *h_features, uint64_t *h_m);
+/* Same features from host keys, computed on the host. */
+ab_status ab_query_features_host(const ab_index *ix, const uint32_t *h_keys, uint32_t n,
+                                 float *h_feature, uint64_t *h_m);
+
+/* ---- synthetic data ------------------------------------------------------- */
+typedef struct ab_dataset ab_dataset;
+/* Filter pairs: nvalues rows, each drawing a row length L then L distinct
+ * keys from Zipf(s) over nkeys keys (ranks mapped to keys by a permutation
+ * seeded with perm_seed).  len_dist 0: L ~ U{lo..hi}; 1: L = round(
+ * LogNormal(ln p1, p2)) clipped to [lo, hi]. */
+ab_status ab_gen_pairs(uint64_t seed, uint64_t perm_seed, uint32_t nvalues, uint32_t nkeys,
+                       int len_dist, double p1, double p2, uint32_t lo, uint32_t hi,
+                       double zipf_s, int nthreads, ab_dataset **out);
+/* Queries: size_dist 0: n = lo; 1: round(LogNormal(ln p1, p2)) clipped to
+ * [lo, hi]; 2: discrete power law P(n) ~ n^-p1 on [lo, hi].  key_dist 0:
+ * distinct uniform keys; 1: distinct Zipf(s)-weighted keys (same permutation
+ * as the pairs for the same perm_seed). */
+ab_status ab_gen_queries(uint64_t seed, uint64_t perm_seed, uint32_t nkeys, uint32_t nq,
+                         int size_dist, double p1, double p2, uint32_t lo, uint32_t hi,
+                         int key_dist, double zipf_s, ab_dataset **out);
+/* Pairs: a = keys, b = values, len = nnz.  Queries: a = keys (concatenated),
+ * offsets = nq+1 prefix, len = total keys. */
+uint64_t ab_dataset_len(const ab_dataset *d);
+uint64_t ab_dataset_count(const ab_dataset *d);
+const uint32_t *ab_dataset_a(const ab_dataset *d);
+const uint32_t *ab_dataset_b(const ab_dataset *d);
+const uint64_t *ab_dataset_offsets(const ab_dataset *d);
+void ab_dataset_free(ab_dataset *d);
+
+/* ---- text loaders --------------------------------------------------------- */
+/* ReadQuery (test/adsfilter_q_all.cu:164-193): returns the key count (or -1)
+ * and fills keys when non-NULL. */
+int64_t ab_read_query(const char *path, uint32_t *keys, uint64_t cap);
+/* ReadCOO into caller buffers (two-call: keys == NULL counts). */
+int64_t ab_read_coo(const char *path, uint32_t *keys, uint32_t *values, uint64_t cap,
+                    uint32_t *max_key, uint32_t *max_value);
+
+/* ---- statistics ----------------------------------------------------------- */
+/* CalculateStats (test/adsfilter_q_all.cu:258-307): out = mean, median,
+ * stddev, min, max, p50, p90, p95, p99 (sorts data in place). */
+void ab_stats(double *data, uint64_t n, double *out9);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADABOOL_H */

@@ -1,0 +1,19 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library)")
+    config.addinivalue_line("markers", "slow: full-size workload (seconds to minutes)")
+
+
+def pytest_runtest_setup(item):
+    # name the case before any GPU work so a failing box still tells which test ran
+    if item.get_closest_marker("gpu") is not None:
+        print(f"\n[gpu-test] {item.nodeid}", flush=True)

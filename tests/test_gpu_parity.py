@@ -1,0 +1,272 @@
+"""GPU parity: the HIP paths (through the C ABI) against the CPU oracle.
+
+Bit-exact on bitmaps (including zero padding) and on ascending id lists."""
+import numpy as np
+import pytest
+
+import adabool_amd as ab
+import oracle
+from adabool_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+PATHS = (ab.AB_PATH_DENSE, ab.AB_PATH_SPARSE, ab.AB_PATH_THRESHOLD)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    import torch
+
+    assert torch.cuda.is_available()
+    k, v = configs.make_filter(configs.C1)
+    ix = ab.Index(k, v, configs.C1.nkeys, configs.C1.nvalues)
+    ctx = ab.Context(ix)
+    csr = oracle.CSR(k, v, configs.C1.nkeys, configs.C1.nvalues)
+    return k, v, ix, ctx, csr
+
+
+def _check(ctx, csr, q, path, what=""):
+    bm, ids, used = ctx.query(q, path)
+    want_bm, want_ids = csr.union(q)
+    assert np.array_equal(bm, want_bm), f"{what} path={path} used={used}: bitmap differs"
+    assert np.array_equal(ids, want_ids), f"{what} path={path} used={used}: ids differ"
+    return used
+
+
+def test_c1_queries_all_paths(c1):
+    _, _, ix, ctx, csr = c1
+    qk, qo = configs.make_queries(configs.C1, configs.C1_Q, 8)
+    for i in range(8):
+        for p in PATHS:
+            _check(ctx, csr, qk[qo[i]:qo[i + 1]], p, f"q{i}")
+
+
+def test_edge_queries(c1):
+    _, _, ix, ctx, csr = c1
+    nk = configs.C1.nkeys
+    cases = {
+        "empty": np.zeros(0, np.uint32),
+        "out_of_range": np.array([nk, nk + 1, 2**32 - 1], np.uint32),
+        "duplicates": np.array([5, 5, 5, 17, 17, 5], np.uint32),
+        "mixed_range": np.array([3, nk + 7, 9, 2**31], np.uint32),
+        "all_keys": np.arange(nk, dtype=np.uint32),
+        "single": np.array([0], np.uint32),
+    }
+    for name, q in cases.items():
+        for p in PATHS:
+            _check(ctx, csr, q, p, name)
+
+
+def test_bitset_and_window_edges():
+    # hand-shaped filter: buckets of every record type at window edges
+    nvalues = 3 * 8192 + 77
+    keys, vals = [], []
+
+    def add(key, vs):
+        for x in vs:
+            keys.append(key)
+            vals.append(x)
+
+    add(0, [0])                                        # ARRAY1 at value 0
+    add(1, [255, 256])                                 # two ARRAY1 across a window edge
+    add(2, [256, 257, 300, 511])                       # ARRAY4
+    add(3, list(range(4096 - 8, 4096)))                # ARRAY8 at a 4096 edge
+    add(4, list(range(8192, 8192 + 14)))               # ARRAY16
+    add(5, list(range(512, 512 + 30)))                 # ARRAY32
+    add(6, list(range(1024, 1024 + 200)))              # BITSET
+    add(7, list(range(0, 256, 2)) + [nvalues - 1])     # BITSET + ARRAY1 at the last value
+    add(8, [nvalues - 1, nvalues - 2, 8191, 8192])     # mixed
+    add(9, list(range(16384, 16384 + 256)))            # full BITSET
+    k = np.array(keys, np.uint32)
+    v = np.array(vals, np.uint32)
+    for ts in (8, 10, 13):
+        ix = ab.Index(k, v, 12, nvalues, ts)
+        ctx = ab.Context(ix)
+        csr = oracle.CSR(k, v, 12, nvalues)
+        for q in ([0], [1], [2], [3], [4], [5], [6], [7], [8], [9], list(range(12)), [6, 9, 11], []):
+            for p in PATHS:
+                _check(ctx, csr, np.array(q, np.uint32), p, f"ts={ts} q={q}")
+
+
+def test_batch_mixed_paths(c1):
+    _, _, ix, ctx, csr = c1
+    nk = configs.C1.nkeys
+    qk, qo = ab.gen_queries(0xADAB0101, nk, 48, size_dist=1, p1=60.0, p2=1.3, lo=1, hi=3000, key_dist=1,
+                            zipf_s=1.0, perm_seed=configs.C1.seed)
+    old = ctx.threshold
+    ctx.set_threshold(0.01)
+    try:
+        bms, ids, counts, paths = ctx.query_batch(qk, qo, ab.AB_PATH_THRESHOLD)
+    finally:
+        ctx.set_threshold(old)
+    assert (paths == ab.AB_PATH_DENSE).any() and (paths == ab.AB_PATH_SPARSE).any(), paths
+    for i in range(48):
+        want_bm, want_ids = csr.union(qk[qo[i]:qo[i + 1]])
+        assert np.array_equal(bms[i], want_bm), f"batch q{i} path {paths[i]}"
+        assert np.array_equal(ids[i], want_ids), f"batch q{i} ids"
+        assert counts[i] == want_ids.size
+
+
+@pytest.mark.parametrize("group", [1, 7, 32, 33, 64])
+def test_batch_dense_groups(c1, group):
+    _, _, ix, ctx, csr = c1
+    qk, qo = ab.gen_queries(0xADAB0102 + group, configs.C1.nkeys, 70, size_dist=1, p1=40.0, p2=1.0, lo=1,
+                            hi=500, key_dist=0)
+    ctx.set_dense_group(group)
+    try:
+        bms, ids, counts, paths = ctx.query_batch(qk, qo, ab.AB_PATH_DENSE)
+    finally:
+        ctx.set_dense_group(64)
+    assert (paths == ab.AB_PATH_DENSE).all()
+    for i in range(qo.size - 1):
+        want_bm, want_ids = csr.union(qk[qo[i]:qo[i + 1]])
+        assert np.array_equal(bms[i], want_bm), f"group {group} q{i}"
+        assert np.array_equal(ids[i], want_ids)
+
+
+@pytest.mark.parametrize("chunk", [256, 1024, 8192])
+def test_batch_sparse_chunks(c1, chunk):
+    _, _, ix, ctx, csr = c1
+    qk, qo = ab.gen_queries(0xADAB0103, configs.C1.nkeys, 40, size_dist=1, p1=30.0, p2=1.5, lo=0, hi=4000,
+                            key_dist=1, zipf_s=1.0, perm_seed=configs.C1.seed)
+    ctx.set_sparse_chunk(chunk)
+    try:
+        bms, ids, counts, paths = ctx.query_batch(qk, qo, ab.AB_PATH_SPARSE)
+    finally:
+        ctx.set_sparse_chunk(4096)
+    for i in range(qo.size - 1):
+        want_bm, want_ids = csr.union(qk[qo[i]:qo[i + 1]])
+        assert np.array_equal(bms[i], want_bm), f"chunk {chunk} q{i}"
+        assert np.array_equal(ids[i], want_ids)
+
+
+@pytest.mark.parametrize("threads", [64, 128, 512, 1024])
+def test_dense_threads(c1, threads):
+    _, _, ix, ctx, csr = c1
+    qk, qo = configs.make_queries(configs.C1, configs.C1_Q, 3)
+    ctx.set_dense_threads(threads)
+    try:
+        for i in range(3):
+            _check(ctx, csr, qk[qo[i]:qo[i + 1]], ab.AB_PATH_DENSE, f"threads {threads}")
+    finally:
+        ctx.set_dense_threads(256)
+
+
+def test_reference_api(c1):
+    import torch
+
+    k, v, _, _, csr = c1
+    nk, nv = configs.C1.nkeys, configs.C1.nvalues
+    lst = ab.IvfList().BuildFrom((k, v))
+    sw = lst.index.stride_words
+    qk, qo = configs.make_queries(configs.C1, configs.C1_Q, 5)
+    qs = [qk[qo[i]:qo[i + 1]] for i in range(5)]
+    # Prepare(keys) + ListUnion_sparse, into a buffer holding garbage
+    d = torch.full((sw,), -1, dtype=torch.int32, device="cuda")
+    lst.Prepare(qs[0]).run(d)
+    lst.sync()
+    assert np.array_equal(d.cpu().numpy().view(np.uint32), csr.union(qs[0])[0])
+    # Prepare(len, bitmap) + ListUnion_dense
+    qb = np.zeros((nk + 7) // 8, np.uint8)
+    for x in qs[1]:
+        qb[x // 8] |= np.uint8(1 << (x % 8))
+    d.fill_(-1)
+    lst.PrepareBitmap(qb.size, qb).run(d)
+    lst.sync()
+    assert np.array_equal(d.cpu().numpy().view(np.uint32), csr.union(qs[1])[0])
+    # PrepareBatchDevice + RunArrays (query-major bitmaps, 16-byte aligned rows)
+    B = 5
+    qbytes = ((nk + 7) // 8 + 15) // 16 * 16
+    hb = np.zeros((B, qbytes), np.uint8)
+    for b in range(B):
+        for x in qs[b]:
+            hb[b, x // 8] |= np.uint8(1 << (x % 8))
+    res_len = sw + 3  # deliberately not a multiple of 4
+    dres = torch.full((B * res_len,), -1, dtype=torch.int32, device="cuda")
+    w = lst.PrepareBatch(B, qbytes, hb)
+    w.RunArrays(dres, res_len)
+    lst.sync()
+    out = dres.cpu().numpy().view(np.uint32).reshape(B, res_len)
+    for b in range(B):
+        assert np.array_equal(out[b, :sw], csr.union(qs[b])[0]), f"batch dense {b}"
+        assert (out[b, sw:] == 0xFFFFFFFF).all(), "wrote past the value space"
+    # BatchListUnion_sparse with interleaved query indices
+    flat = np.concatenate(qs)
+    qidx = np.concatenate([np.full(q.size, b, np.uint32) for b, q in enumerate(qs)])
+    perm = np.random.default_rng(1).permutation(flat.size)
+    dres.fill_(-1)
+    lst.BatchListUnionSparse(flat[perm], qidx[perm], B, dres, res_len * 4)
+    lst.sync()
+    out = dres.cpu().numpy().view(np.uint32).reshape(B, res_len)
+    for b in range(B):
+        assert np.array_equal(out[b, :sw], csr.union(qs[b])[0]), f"batch sparse {b}"
+
+
+def test_features_device_matches_host(c1):
+    import torch
+
+    _, _, ix, ctx, _ = c1
+    qk, qo = ab.gen_queries(0xADAB0104, configs.C1.nkeys, 20, size_dist=1, p1=50.0, p2=1.0, lo=0, hi=2000,
+                            key_dist=1, zipf_s=1.0, perm_seed=configs.C1.seed)
+    dk = torch.from_numpy(qk.view(np.int32)).cuda()
+    f, m = ctx.features(dk, qo)
+    for i in range(20):
+        fh, mh = ix.features_host(qk[qo[i]:qo[i + 1]])
+        assert m[i] == mh
+        np.testing.assert_allclose(f[i], fh, rtol=1e-5, atol=1e-6)
+
+
+def test_compact_ids_direct(c1):
+    import torch
+
+    _, _, ix, ctx, csr = c1
+    rng = np.random.default_rng(7)
+    sw = ix.stride_words
+    nv = configs.C1.nvalues
+    raw = rng.integers(0, 2**32, size=(3, sw), dtype=np.uint64).astype(np.uint32)
+    raw[0] = 0
+    raw[2, : sw // 2] = 0
+    # padding bits beyond nvalues are zero in every engine bitmap
+    last = nv // 32
+    raw[:, last + 1:] = 0
+    if nv % 32:
+        raw[:, last] &= np.uint32((1 << (nv % 32)) - 1)
+    else:
+        raw[:, last] = 0
+    d = torch.from_numpy(raw.reshape(-1).view(np.int32)).cuda()
+    ids = torch.zeros(3 * nv, dtype=torch.int32, device="cuda")
+    cnt = torch.zeros(3, dtype=torch.int32, device="cuda")
+    ab._check(ab._lib.ab_compact_ids(ctx._h, d.data_ptr(), 3, ids.data_ptr(), nv, cnt.data_ptr()))
+    ctx.sync()
+    c = cnt.cpu().numpy()
+    ih = ids.cpu().numpy().view(np.uint32).reshape(3, nv)
+    for q in range(3):
+        want = oracle.collect_ids(raw[q], sw * 32, nv)
+        assert c[q] == want.size
+        assert np.array_equal(ih[q, : c[q]], want)
+
+
+@pytest.mark.slow
+def test_c2_full_size_properties():
+    """C2 at full size: dense == sparse == threshold bitmaps on sampled
+    queries, counts and ids equal to the oracle."""
+    import torch
+
+    spec = configs.C2
+    k, v = configs.make_filter(spec)
+    ix = ab.Index(k, v, spec.nkeys, spec.nvalues)
+    ctx = ab.Context(ix)
+    qk, qo = configs.make_queries(spec, configs.C2_Q, 24)
+    csr = oracle.CSR(k, v, spec.nkeys, spec.nvalues)
+    del k, v
+    res = {}
+    for p in PATHS:
+        res[p] = ctx.query_batch(qk, qo, p)
+    for i in range(24):
+        want_bm, want_ids = csr.union(qk[qo[i]:qo[i + 1]])
+        for p in PATHS:
+            bms, ids, counts, paths = res[p]
+            assert np.array_equal(bms[i], want_bm), f"C2 q{i} path {p}"
+            assert counts[i] == want_ids.size
+            if i < 6:
+                assert np.array_equal(ids[i], want_ids)
