@@ -96,6 +96,8 @@ _SIGS = {
     "ab_ctx_stream": (_VP, [_VP]),
     "ab_ctx_set_threshold": (C.c_int, [_VP, C.c_double]),
     "ab_ctx_get_threshold": (C.c_double, [_VP]),
+    "ab_ctx_set_batch_threshold": (C.c_int, [_VP, C.c_double]),
+    "ab_ctx_get_batch_threshold": (C.c_double, [_VP]),
     "ab_ctx_set_dense_group": (C.c_int, [_VP, C.c_uint32]),
     "ab_ctx_set_dense_threads": (C.c_int, [_VP, C.c_uint32]),
     "ab_ctx_set_sparse_chunk": (C.c_int, [_VP, C.c_uint32]),
@@ -154,7 +156,15 @@ def load_library(path: Optional[str] = None):
     return lib
 
 
-_OPTIONAL_SIGS: dict = {}
+_OPTIONAL_SIGS: dict = {
+    "ab_mlp_nparams": (C.c_uint32, []),
+    "ab_mlp_set": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "ab_mlp_get": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "ab_mlp_clear": (C.c_int, [_VP]),
+    "ab_mlp_predict": (C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
+    "ab_mlp_grad": (C.c_int, [_VP, _VP, _VP, C.c_uint32, _VP, _VP]),
+    "ab_mlp_train": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _VP]),
+}
 
 
 def _check(st: int):
@@ -385,6 +395,13 @@ class Context:
     def threshold(self) -> float:
         return _lib.ab_ctx_get_threshold(self._h)
 
+    def set_batch_threshold(self, tau: float):
+        _check(_lib.ab_ctx_set_batch_threshold(self._h, tau))
+
+    @property
+    def batch_threshold(self) -> float:
+        return _lib.ab_ctx_get_batch_threshold(self._h)
+
     def set_dense_group(self, w: int):
         _check(_lib.ab_ctx_set_dense_group(self._h, w))
 
@@ -470,6 +487,59 @@ class Context:
             for q in range(nq):
                 out_ids.append(ih[q, : counts[q]].cpu().numpy().view(np.uint32))
         return bm_h, out_ids, counts, paths
+
+    # cost model -----------------------------------------------------------
+    def set_model(self, params, mean, std):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        mu = np.ascontiguousarray(mean, dtype=np.float32)
+        sd = np.ascontiguousarray(std, dtype=np.float32)
+        if p.size != _lib.ab_mlp_nparams() or mu.size != NFEATURES or sd.size != NFEATURES:
+            raise ValueError("cost model shape mismatch")
+        _check(_lib.ab_mlp_set(self._h, _ptr(p), _ptr(mu), _ptr(sd)))
+
+    def get_model(self):
+        p = np.zeros(_lib.ab_mlp_nparams(), np.float32)
+        mu = np.zeros(NFEATURES, np.float32)
+        sd = np.zeros(NFEATURES, np.float32)
+        _check(_lib.ab_mlp_get(self._h, _ptr(p), _ptr(mu), _ptr(sd)))
+        return p, mu, sd
+
+    def clear_model(self):
+        _check(_lib.ab_mlp_clear(self._h))
+
+    def load_model(self, path: str) -> dict:
+        import json
+        d = json.load(open(path))
+        self.set_model(d["params"], d["mean"], d["std"])
+        return d
+
+    def save_model(self, path: str, meta: Optional[dict] = None):
+        import json
+        p, mu, sd = self.get_model()
+        json.dump({"dims": [NFEATURES, 64, 64, 2], "params": p.tolist(), "mean": mu.tolist(), "std": sd.tolist(),
+                   "meta": meta or {}}, open(path, "w"))
+
+    def predict(self, x):
+        xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEATURES)
+        y = np.zeros((xa.shape[0], 2), np.float32)
+        sel = np.zeros(max(xa.shape[0], 1), np.uint8)
+        _check(_lib.ab_mlp_predict(self._h, _ptr(xa), xa.shape[0], _ptr(y), _ptr(sel)))
+        return y, sel[: xa.shape[0]]
+
+    def model_grad(self, x, y):
+        xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEATURES)
+        ya = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, 2)
+        g = np.zeros(_lib.ab_mlp_nparams(), np.float32)
+        loss = C.c_float()
+        _check(_lib.ab_mlp_grad(self._h, _ptr(xa), _ptr(ya), xa.shape[0], _ptr(g), C.byref(loss)))
+        return loss.value, g
+
+    def train_model(self, x, y, epochs: int = 30, batch: int = 256, lr: float = 1e-3, seed: int = 1):
+        xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEATURES)
+        ya = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, 2)
+        curve = np.zeros(max(epochs, 1), np.float32)
+        _check(_lib.ab_mlp_train(self._h, _ptr(xa), _ptr(ya), xa.shape[0], epochs, batch, lr, seed, _ptr(curve)))
+        return curve[:epochs]
 
     def features(self, d_keys, qoff) -> Tuple[np.ndarray, np.ndarray]:
         off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))

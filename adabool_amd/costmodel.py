@@ -1,0 +1,179 @@
+"""Learned dispatch: timing collection, training and regret of the cost model.
+
+Labels come from the method itself, exactly as the reference harness does it
+(test/adsfilter_q_all.cu:414-438): every query is run through BOTH paths and
+each is timed.  Here each run is one synchronized single-query call
+(enqueue -> stream synchronize, host wall clock), best of `repeats`.  The
+model (12 features -> 64 -> 64 -> 2, log milliseconds of dense and sparse) is
+trained by the library's own HIP kernels (ab_mlp_train); the threshold rule
+(dense iff M(q) >= tau * R, tau fit on the same training timings) is both the
+fallback and the yardstick: regret = sum of the chosen path's time over the
+sum of the per-query best time, minus one.
+
+Run as a module on a GPU host to produce the committed model:
+    python -m adabool_amd.costmodel --out adabool_amd/models/cost_model_c4.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from dataclasses import replace
+from typing import List
+
+import numpy as np
+
+from . import AB_PATH_DENSE, AB_PATH_SPARSE, Context, Index
+from . import configs
+
+# Seeded indices spanning 1M..16M values and Zipf s 0.8..1.3 (C2-like rows).
+TRAIN_INDICES = [
+    (1_000_000, 0.8), (2_000_000, 1.3), (4_000_000, 1.0), (8_000_000, 1.2), (16_000_000, 1.1), (2_000_000, 0.9),
+]
+HELDOUT_INDEX = (4_000_000, 1.1)  # the C2 shape, held out of training
+
+
+def filter_spec(nvalues: int, s: float, k: int) -> configs.FilterSpec:
+    return replace(configs.C2, name=f"C4-{nvalues // 1_000_000}M-s{s}", seed=0xADAB0400 + k, nvalues=nvalues,
+                   nkeys=nvalues // 4, zipf_s=s)
+
+
+def query_mix(spec: configs.FilterSpec, nq: int, seed: int):
+    """Half power-law sizes (C3), half lognormal (C2); Zipf-weighted keys
+    with the filter's own exponent, plus a uniform-key slice."""
+    from . import gen_queries
+
+    n1, n2 = nq // 2, nq // 4
+    n3 = nq - n1 - n2
+    parts = [
+        gen_queries(seed, spec.nkeys, n1, size_dist=2, p1=1.5, lo=1, hi=50_000, key_dist=1, zipf_s=spec.zipf_s,
+                    perm_seed=spec.seed),
+        gen_queries(seed + 1, spec.nkeys, n2, size_dist=1, p1=200.0, p2=1.2, lo=8, hi=20_000, key_dist=1,
+                    zipf_s=spec.zipf_s, perm_seed=spec.seed),
+        gen_queries(seed + 2, spec.nkeys, n3, size_dist=1, p1=500.0, p2=1.5, lo=1, hi=50_000, key_dist=0),
+    ]
+    keys = np.concatenate([p[0] for p in parts])
+    off = [np.zeros(1, np.uint64)]
+    base = 0
+    for k, o in parts:
+        off.append(o[1:] + base)
+        base += int(o[-1])
+    return keys, np.concatenate(off)
+
+
+def time_both(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, repeats: int = 2):
+    """Per query: features, M, and best-of-repeats wall ms of dense and sparse."""
+    import torch
+
+    nq = qoff.size - 1
+    sw = ix.stride_words
+    nv = ix.info["nvalues"]
+    bm = torch.empty(sw, dtype=torch.int32, device="cuda")
+    ids = torch.empty(nv, dtype=torch.int32, device="cuda")
+    cnt = torch.empty(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    feats = np.zeros((nq, 12), np.float32)
+    M = np.zeros(nq, np.uint64)
+    t = np.full((nq, 2), np.inf)
+    for i in range(nq):
+        q = keys[int(qoff[i]):int(qoff[i + 1])]
+        feats[i], M[i] = ix.features_host(q)
+        for r in range(repeats):
+            for j, p in enumerate((AB_PATH_DENSE, AB_PATH_SPARSE)):
+                s0 = time.perf_counter()
+                ctx.query_device(q, p, bm, ids, cnt)
+                ctx.sync()
+                t[i, j] = min(t[i, j], (time.perf_counter() - s0) * 1e3)
+    return feats, M, t
+
+
+def fit_threshold(M: np.ndarray, R: np.ndarray, t: np.ndarray) -> float:
+    """tau minimizing total time of the rule dense iff M >= tau * R."""
+    frac = M.astype(np.float64) / R.astype(np.float64)
+    cands = np.unique(np.concatenate([[0.0, 1.0 + 1e-9], frac]))
+    order = np.argsort(frac)
+    fs, td, tsp = frac[order], t[order, 0], t[order, 1]
+    # choosing tau between fs[k-1] and fs[k]: first k sparse, rest dense
+    cs = np.concatenate([[0.0], np.cumsum(tsp)])
+    cd = np.concatenate([np.cumsum(td[::-1])[::-1], [0.0]])
+    total = cs + cd
+    k = int(np.argmin(total))
+    if k == 0:
+        return float(fs[0]) if fs.size else 0.0
+    if k >= fs.size:
+        return float(fs[-1] + 1e-9)
+    del cands
+    return float((fs[k - 1] + fs[k]) / 2)
+
+
+def regret(choice_dense: np.ndarray, t: np.ndarray) -> float:
+    chosen = np.where(choice_dense, t[:, 0], t[:, 1])
+    return float(chosen.sum() / np.minimum(t[:, 0], t[:, 1]).sum() - 1.0)
+
+
+def main(argv: List[str] = None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default="gpurun_out/cost_model_c4.json")
+    ap.add_argument("--data-out", default="gpurun_out/cost_timings_c4.npz")
+    ap.add_argument("--queries", type=int, default=2000, help="timed queries per index")
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--repeats", type=int, default=2)
+    ap.add_argument("--scale", type=float, default=1.0, help="index size factor (for quick runs)")
+    a = ap.parse_args(argv)
+
+    import torch
+
+    assert torch.cuda.is_available()
+    rows = []
+    specs = [(filter_spec(int(nv * a.scale), s, k), False) for k, (nv, s) in enumerate(TRAIN_INDICES)]
+    specs.append((filter_spec(int(HELDOUT_INDEX[0] * a.scale), HELDOUT_INDEX[1], 99), True))
+    for spec, held in specs:
+        t0 = time.time()
+        pk, pv = configs.make_filter(spec)
+        ix = Index(pk, pv, spec.nkeys, spec.nvalues)
+        del pk, pv
+        ctx = Context(ix)
+        keys, qoff = query_mix(spec, a.queries, spec.seed + 17)
+        f, M, t = time_both(ctx, ix, keys, qoff, a.repeats)
+        R = np.full(M.size, ix.info["nrec_total"], np.uint64)
+        rows.append((f, M, R, t, np.full(M.size, held)))
+        print(f"[costmodel] {spec.name}: R={ix.info['nrec_total']} {a.queries} queries timed in "
+              f"{time.time() - t0:.1f}s; dense wins {(t[:, 0] < t[:, 1]).mean():.2f}", file=sys.stderr, flush=True)
+        ctx.close()
+        ix.close()
+    F = np.concatenate([r[0] for r in rows])
+    M = np.concatenate([r[1] for r in rows])
+    R = np.concatenate([r[2] for r in rows])
+    T = np.concatenate([r[3] for r in rows])
+    H = np.concatenate([r[4] for r in rows])
+    os.makedirs(os.path.dirname(os.path.abspath(a.data_out)), exist_ok=True)
+    np.savez_compressed(a.data_out, features=F, M=M, R=R, t_ms=T, heldout=H)
+    tr, te = ~H, H
+    tau = fit_threshold(M[tr], R[tr], T[tr])
+    # train on the GPU with the library's kernels (any index's context works)
+    spec = filter_spec(int(TRAIN_INDICES[0][0] * a.scale), TRAIN_INDICES[0][1], 0)
+    pk, pv = configs.make_filter(spec)
+    ix = Index(pk, pv, spec.nkeys, spec.nvalues)
+    ctx = Context(ix)
+    Y = np.log(T).astype(np.float32)
+    curve = ctx.train_model(F[tr], Y[tr], epochs=a.epochs, batch=256, lr=2e-3, seed=7)
+    pred, sel = ctx.predict(F)
+    learned = sel == AB_PATH_DENSE
+    thr = (M.astype(np.float64) >= tau * R.astype(np.float64))
+    res = {
+        "tau": tau,
+        "train_regret_threshold": regret(thr[tr], T[tr]), "train_regret_learned": regret(learned[tr], T[tr]),
+        "heldout_regret_threshold": regret(thr[te], T[te]), "heldout_regret_learned": regret(learned[te], T[te]),
+        "heldout_queries": int(te.sum()), "train_queries": int(tr.sum()),
+        "final_loss": float(curve[-1]), "first_loss": float(curve[0]),
+    }
+    print(json.dumps(res), flush=True)
+    os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
+    ctx.save_model(a.out, meta=dict(res, trained_on=[list(x) for x in TRAIN_INDICES], heldout=list(HELDOUT_INDEX),
+                                    labels="log ms, single-query synchronized wall time, best of repeats"))
+
+
+if __name__ == "__main__":
+    main()

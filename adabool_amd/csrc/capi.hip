@@ -455,6 +455,18 @@ extern "C" ab_status ab_ctx_set_threshold(ab_ctx *c, double tau)
 
 extern "C" double ab_ctx_get_threshold(const ab_ctx *c) { return c ? c->tau : 0; }
 
+extern "C" ab_status ab_ctx_set_batch_threshold(ab_ctx *c, double tau)
+{
+    if (!c || !(tau >= 0)) {
+        set_error("ab_ctx_set_batch_threshold: bad argument");
+        return AB_INVALID_ARG;
+    }
+    c->tau_batch = tau;
+    return AB_OK;
+}
+
+extern "C" double ab_ctx_get_batch_threshold(const ab_ctx *c) { return c ? c->tau_batch : 0; }
+
 extern "C" ab_status ab_ctx_set_dense_group(ab_ctx *c, uint32_t w)
 {
     if (!c || w < 1 || w > (uint32_t)kMaxGroup) {
@@ -626,8 +638,8 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
             if (learned) {
                 AB_TRY(mlp_decide(c, (const float *)c->feat.p, nq, (uint8_t *)c->sel.p));
             } else {
-                launch_decide_threshold((const uint64_t *)c->M.p, nq, c->tau * (double)ix->R, (uint8_t *)c->sel.p,
-                                        s);
+                const double tau = nq > 1 ? c->tau_batch : c->tau;
+                launch_decide_threshold((const uint64_t *)c->M.p, nq, tau * (double)ix->R, (uint8_t *)c->sel.p, s);
             }
         }
         HIP_TRY(hipMemcpyAsync(hsel, c->sel.p, nq, hipMemcpyDeviceToHost, s));
@@ -708,14 +720,15 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
             }
             {
                 TimedSpan ts(c, AB_K_DENSE_MASK);
-                launch_mask_keys(mode, a.d_keys, qoff_d, g, mask, ix->nkeys, 0, s);
+                launch_mask_keys(mode, a.d_keys, qoff_d, g, mask, (uint32_t *)c->qbm.p, ix->nkeys, 0, s);
             }
             {
                 TimedSpan ts(c, AB_K_DENSE_TILES);
-                HIP_TRY(launch_dense_tiles(mode, c->dense_threads, d, mask, g, bm, out_stride, out_words, s));
+                HIP_TRY(launch_dense_tiles(mode, c->dense_threads, d, mask, (const uint32_t *)c->qbm.p, g, bm, out_stride,
+                                           out_words, s));
             }
             TimedSpan ts(c, AB_K_DENSE_MASK);
-            launch_mask_keys(mode, a.d_keys, qoff_d, g, mask, ix->nkeys, 1, s);
+            launch_mask_keys(mode, a.d_keys, qoff_d, g, mask, (uint32_t *)c->qbm.p, ix->nkeys, 1, s);
         }
     }
 
@@ -822,7 +835,8 @@ extern "C" ab_status ab_list_union_dense(ab_ctx *c, uint64_t len_bytes, const ui
     g.B = 1;
     g.q[0] = 0;
     const uint64_t w = ix->stride / 4;
-    HIP_TRY(launch_dense_tiles(kModeBitmap, c->dense_threads, ix->dev, c->qbm.p, g, d_res, w, w, c->stream));
+    HIP_TRY(launch_dense_tiles(kModeBitmap, c->dense_threads, ix->dev, c->qbm.p, (const uint32_t *)c->qbm.p, g, d_res, w,
+                               w, c->stream));
     HIP_TRY(hipMemsetAsync(c->qbm.p, 0, qbytes, c->stream));
     return AB_OK;
 }
@@ -849,12 +863,14 @@ extern "C" ab_status ab_list_union_dense_batch(ab_ctx *c, uint32_t B, uint64_t l
         DenseGroup g;
         g.B = std::min(G, B - b0);
         for (uint32_t b = 0; b < g.B; ++b) g.q[b] = b0 + b;
-        launch_mask_from_bitmaps((const uint32_t *)d_bitmaps, len_bytes / 4, b0, g.B, c->mask.p, ix->nkeys,
-                                 c->stream);
-        HIP_TRY(launch_dense_tiles(kModeMask64, c->dense_threads, ix->dev, c->mask.p, g, d_res, res_len_u32,
-                                   out_words, c->stream));
+        launch_mask_from_bitmaps((const uint32_t *)d_bitmaps, len_bytes / 4, b0, g.B, c->mask.p,
+                                 (uint32_t *)c->qbm.p, ix->nkeys, c->stream);
+        HIP_TRY(launch_dense_tiles(kModeMask64, c->dense_threads, ix->dev, c->mask.p, (const uint32_t *)c->qbm.p, g,
+                                   d_res, res_len_u32, out_words, c->stream));
     }
+    // both filters back to all-zero for the key-list paths
     HIP_TRY(hipMemsetAsync(c->mask.p, 0, (size_t)ix->nkeys * 8, c->stream));
+    HIP_TRY(hipMemsetAsync(c->qbm.p, 0, c->qbm.cap, c->stream));
     return AB_OK;
 }
 

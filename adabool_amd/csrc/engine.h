@@ -70,7 +70,8 @@ struct ab_ctx {
     ab_index *ix = nullptr;
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    double tau = 0.05;
+    double tau = 0.05;          // threshold rule for single queries
+    double tau_batch = 0.003;   // threshold rule inside batches (dense passes are shared)
     uint32_t group = 64;
     uint32_t dense_threads = 256;
     uint32_t sparse_chunk = 4096;

@@ -41,17 +41,19 @@ void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *pa
 void launch_sparse_union(const DevIndex &ix, const uint32_t *keys, const uint32_t *kq,
                          const uint64_t *koff, uint64_t K, uint32_t *out, uint64_t out_stride,
                          uint32_t chunk, uint32_t grid, hipStream_t s);
-// Sets bit b of mask[k] (or of bitmap word k>>5 for B==1 with bitmap mode)
-// for every key k of query g.q[b]; clear=1 writes zeros at the same places.
+// For every key k of query g.q[b]: sets bit k of anybm (1 bit per key) and,
+// in the mask modes, bit b of mask[k]; clear=1 writes zeros at the same
+// places, returning both buffers to all-zero.
 void launch_mask_keys(int mode, const uint32_t *keys, const uint64_t *qoff, const DenseGroup &g,
-                      void *mask, uint32_t nkeys, int clear, hipStream_t s);
+                      void *mask, uint32_t *anybm, uint32_t nkeys, int clear, hipStream_t s);
+// Writes mask[k] (u64) and anybm for every k < nkeys from B query bitmaps.
 void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_t b0, uint32_t B,
-                              void *mask, uint32_t nkeys, hipStream_t s);
+                              void *mask, uint32_t *anybm, uint32_t nkeys, hipStream_t s);
 // dynamic LDS bytes of the dense kernel for a group of B queries
 uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift);
 hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask,
-                              const DenseGroup &g, uint32_t *out, uint64_t out_stride,
-                              uint64_t out_words, hipStream_t s);
+                              const uint32_t *anybm, const DenseGroup &g, uint32_t *out,
+                              uint64_t out_stride, uint64_t out_words, hipStream_t s);
 constexpr uint32_t kCompactWords = 2048;
 void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
                      uint32_t *ccnt, hipStream_t s);

@@ -396,7 +396,8 @@ void launch_sparse_union(const DevIndex &ix, const uint32_t *keys, const uint32_
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(256) void k_mask_keys(const uint32_t *__restrict__ keys, const uint64_t *__restrict__ qoff,
-                                                   DenseGroup g, void *mask, uint32_t nkeys, int clear)
+                                                   DenseGroup g, void *mask, uint32_t *anybm, uint32_t nkeys,
+                                                   int clear)
 {
     const uint32_t b = blockIdx.y;
     const uint32_t q = g.q[b];
@@ -404,13 +405,14 @@ __global__ __launch_bounds__(256) void k_mask_keys(const uint32_t *__restrict__ 
     for (uint64_t j = a + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < e; j += (uint64_t)gridDim.x * 256) {
         const uint32_t k = keys[j];
         if (k >= nkeys) continue;
-        if constexpr (MODE == kModeBitmap) {
-            uint32_t *m = reinterpret_cast<uint32_t *>(mask) + (k >> 5);
-            if (clear) *m = 0; else atomicOr(m, 1u << (k & 31));
-        } else if constexpr (MODE == kModeMask32) {
+        // the 1-bit-per-key bitmap: the query itself in bitmap mode, the
+        // union of the group's queries (a filter in front of mask) otherwise
+        uint32_t *w = anybm + (k >> 5);
+        if (clear) *w = 0; else atomicOr(w, 1u << (k & 31));
+        if constexpr (MODE == kModeMask32) {
             uint32_t *m = reinterpret_cast<uint32_t *>(mask) + k;
             if (clear) *m = 0; else atomicOr(m, 1u << b);
-        } else {
+        } else if constexpr (MODE == kModeMask64) {
             unsigned long long *m = reinterpret_cast<unsigned long long *>(mask) + k;
             if (clear) *m = 0; else atomicOr(m, 1ull << b);
         }
@@ -418,38 +420,50 @@ __global__ __launch_bounds__(256) void k_mask_keys(const uint32_t *__restrict__ 
 }
 
 void launch_mask_keys(int mode, const uint32_t *keys, const uint64_t *qoff, const DenseGroup &g, void *mask,
-                      uint32_t nkeys, int clear, hipStream_t s)
+                      uint32_t *anybm, uint32_t nkeys, int clear, hipStream_t s)
 {
     dim3 grid(8, g.B);
     if (mode == kModeBitmap)
-        hipLaunchKernelGGL(k_mask_keys<kModeBitmap>, grid, dim3(256), 0, s, keys, qoff, g, mask, nkeys, clear);
+        hipLaunchKernelGGL(k_mask_keys<kModeBitmap>, grid, dim3(256), 0, s, keys, qoff, g, mask, anybm, nkeys, clear);
     else if (mode == kModeMask32)
-        hipLaunchKernelGGL(k_mask_keys<kModeMask32>, grid, dim3(256), 0, s, keys, qoff, g, mask, nkeys, clear);
+        hipLaunchKernelGGL(k_mask_keys<kModeMask32>, grid, dim3(256), 0, s, keys, qoff, g, mask, anybm, nkeys, clear);
     else
-        hipLaunchKernelGGL(k_mask_keys<kModeMask64>, grid, dim3(256), 0, s, keys, qoff, g, mask, nkeys, clear);
+        hipLaunchKernelGGL(k_mask_keys<kModeMask64>, grid, dim3(256), 0, s, keys, qoff, g, mask, anybm, nkeys, clear);
 }
 
-// mask[k] bit b = bit k of bitmap (b0 + b), for k < nkeys (every entry written)
-__global__ void k_mask_from_bitmaps(const uint32_t *__restrict__ bms, uint64_t len_u32, uint32_t b0, uint32_t B,
-                                    unsigned long long *mask, uint32_t nkeys)
+// mask[k] bit b = bit k of bitmap (b0 + b), and anybm bit k = (mask[k] != 0),
+// for every k < nkeys (every entry written; waves cover 64 aligned keys).
+__global__ __launch_bounds__(256) void k_mask_from_bitmaps(const uint32_t *__restrict__ bms, uint64_t len_u32,
+                                                           uint32_t b0, uint32_t B, unsigned long long *mask,
+                                                           uint32_t *anybm, uint32_t nkeys)
 {
-    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < nkeys; k += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t wave0 = ((uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t kb = wave0; kb < nkeys; kb += stride) {
+        const uint64_t k = kb + lane;
         unsigned long long m = 0;
-        const uint64_t w = k >> 5;
-        if (w < len_u32)
-            for (uint32_t b = 0; b < B; ++b) m |= (unsigned long long)((bms[(uint64_t)(b0 + b) * len_u32 + w] >> (k & 31)) & 1u) << b;
-        mask[k] = m;
+        if (k < nkeys) {
+            const uint64_t w = k >> 5;
+            if (w < len_u32)
+                for (uint32_t b = 0; b < B; ++b)
+                    m |= (unsigned long long)((bms[(uint64_t)(b0 + b) * len_u32 + w] >> (k & 31)) & 1u) << b;
+            mask[k] = m;
+        }
+        const unsigned long long any = __ballot(m != 0);
+        if (lane == 0) anybm[kb >> 5] = (uint32_t)any;
+        if (lane == 32 && kb + 32 < nkeys) anybm[(kb >> 5) + 1] = (uint32_t)(any >> 32);
     }
 }
 
 void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_t b0, uint32_t B, void *mask,
-                              uint32_t nkeys, hipStream_t s)
+                              uint32_t *anybm, uint32_t nkeys, hipStream_t s)
 {
     uint64_t grid = ((uint64_t)nkeys + 255) / 256;
     if (grid > 8192) grid = 8192;
     if (!grid) return;
     hipLaunchKernelGGL(k_mask_from_bitmaps, dim3((uint32_t)grid), dim3(256), 0, s, bitmaps, len_u32, b0, B,
-                       (unsigned long long *)mask, nkeys);
+                       (unsigned long long *)mask, anybm, nkeys);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,15 +472,19 @@ void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_
 template <int MODE> struct MaskOf { using T = uint32_t; };
 template <> struct MaskOf<kModeMask64> { using T = unsigned long long; };
 
+// Query bits of key k: the 1-bit-per-key bitmap answers alone for one query
+// and filters the per-key mask loads (8 bytes per key) for a group.
 template <int MODE>
-__device__ __forceinline__ typename MaskOf<MODE>::T probe(const void *__restrict__ m, uint32_t k)
+__device__ __forceinline__ typename MaskOf<MODE>::T probe(const void *__restrict__ m,
+                                                          const uint32_t *__restrict__ anybm, uint32_t k)
 {
+    const uint32_t hit = (anybm[k >> 5] >> (k & 31)) & 1u;
     if constexpr (MODE == kModeBitmap)
-        return (reinterpret_cast<const uint32_t *>(m)[k >> 5] >> (k & 31)) & 1u;
+        return hit;
     else if constexpr (MODE == kModeMask32)
-        return reinterpret_cast<const uint32_t *>(m)[k];
+        return hit ? reinterpret_cast<const uint32_t *>(m)[k] : 0u;
     else
-        return reinterpret_cast<const unsigned long long *>(m)[k];
+        return hit ? reinterpret_cast<const unsigned long long *>(m)[k] : 0ull;
 }
 
 template <int MODE, int T>
@@ -503,7 +521,8 @@ __device__ __forceinline__ void dense_emit_block(const DevIndex &ix, uint32_t re
 }
 
 template <int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__restrict__ mask, DenseGroup g,
+__global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__restrict__ mask,
+                                                    const uint32_t *__restrict__ anybm, DenseGroup g,
                                                     uint32_t *__restrict__ out, uint64_t out_stride,
                                                     uint64_t out_words)
 {
@@ -548,7 +567,7 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
             const uint32_t r = (uint32_t)s * 64 + lane;
-            mk[s] = r < nv ? probe<MODE>(mask, mt.x + key[s]) : (M)0;
+            mk[s] = r < nv ? probe<MODE>(mask, anybm, mt.x + key[s]) : (M)0;
         }
         switch (type) {
         case AB_ARRAY1: dense_emit_block<MODE, AB_ARRAY1>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
@@ -571,28 +590,32 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
 uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift) { return B * (1u << (tile_shift - 5)) * 4u; }
 
 template <int MODE>
-static hipError_t launch_dense_mode(uint32_t threads, const DevIndex &ix, const void *mask, const DenseGroup &g,
-                                    uint32_t *out, uint64_t out_stride, uint64_t out_words, hipStream_t s)
+static hipError_t launch_dense_mode(uint32_t threads, const DevIndex &ix, const void *mask, const uint32_t *anybm,
+                                    const DenseGroup &g, uint32_t *out, uint64_t out_stride, uint64_t out_words,
+                                    hipStream_t s)
 {
     const uint32_t lds = dense_lds_bytes(g.B, ix.tile_shift);
     dim3 grid(ix.ntiles);
     switch (threads) {
-    case 64: hipLaunchKernelGGL((k_dense_tiles<MODE, 64>), grid, dim3(64), lds, s, ix, mask, g, out, out_stride, out_words); break;
-    case 128: hipLaunchKernelGGL((k_dense_tiles<MODE, 128>), grid, dim3(128), lds, s, ix, mask, g, out, out_stride, out_words); break;
-    case 512: hipLaunchKernelGGL((k_dense_tiles<MODE, 512>), grid, dim3(512), lds, s, ix, mask, g, out, out_stride, out_words); break;
-    case 1024: hipLaunchKernelGGL((k_dense_tiles<MODE, 1024>), grid, dim3(1024), lds, s, ix, mask, g, out, out_stride, out_words); break;
-    default: hipLaunchKernelGGL((k_dense_tiles<MODE, 256>), grid, dim3(256), lds, s, ix, mask, g, out, out_stride, out_words); break;
+    case 64: hipLaunchKernelGGL((k_dense_tiles<MODE, 64>), grid, dim3(64), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
+    case 128: hipLaunchKernelGGL((k_dense_tiles<MODE, 128>), grid, dim3(128), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
+    case 512: hipLaunchKernelGGL((k_dense_tiles<MODE, 512>), grid, dim3(512), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
+    case 1024: hipLaunchKernelGGL((k_dense_tiles<MODE, 1024>), grid, dim3(1024), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
+    default: hipLaunchKernelGGL((k_dense_tiles<MODE, 256>), grid, dim3(256), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask, const DenseGroup &g,
-                              uint32_t *out, uint64_t out_stride, uint64_t out_words, hipStream_t s)
+hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask, const uint32_t *anybm,
+                              const DenseGroup &g, uint32_t *out, uint64_t out_stride, uint64_t out_words,
+                              hipStream_t s)
 {
     if (!ix.ntiles || !g.B) return hipSuccess;
-    if (mode == kModeBitmap) return launch_dense_mode<kModeBitmap>(threads, ix, mask, g, out, out_stride, out_words, s);
-    if (mode == kModeMask32) return launch_dense_mode<kModeMask32>(threads, ix, mask, g, out, out_stride, out_words, s);
-    return launch_dense_mode<kModeMask64>(threads, ix, mask, g, out, out_stride, out_words, s);
+    if (mode == kModeBitmap)
+        return launch_dense_mode<kModeBitmap>(threads, ix, mask, anybm, g, out, out_stride, out_words, s);
+    if (mode == kModeMask32)
+        return launch_dense_mode<kModeMask32>(threads, ix, mask, anybm, g, out, out_stride, out_words, s);
+    return launch_dense_mode<kModeMask64>(threads, ix, mask, anybm, g, out, out_stride, out_words, s);
 }
 
 // ---------------------------------------------------------------------------

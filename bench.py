@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--baseline-queries", type=int, default=256)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tau", type=float, default=None, help="single-query threshold")
+    ap.add_argument("--tau-batch", type=float, default=None, help="batch threshold")
+    ap.add_argument("--calib-copy", type=int, default=0, help="MiB copied once by a plain copy kernel (PMC scale)")
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     args = ap.parse_args()
 
@@ -132,6 +135,19 @@ def main():
         ctx.load_model(args.model)
         dispatch = "learned"
 
+    if args.tau is not None:
+        ctx.set_threshold(args.tau)
+    if args.tau_batch is not None:
+        ctx.set_batch_threshold(args.tau_batch)
+    if args.calib_copy:
+        n = args.calib_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_copy} MiB read + {args.calib_copy} MiB written")
+        del src, dst
     sw = ix.stride_words
     nv = spec.nvalues
     d_keys = torch.from_numpy(qk.view(np.int32)).cuda()

@@ -139,6 +139,10 @@ void *ab_ctx_stream(ab_ctx *ctx);
 /* Threshold rule: dense iff M(q) >= tau * R (M = records the query matches). */
 ab_status ab_ctx_set_threshold(ab_ctx *ctx, double tau);
 double ab_ctx_get_threshold(const ab_ctx *ctx);
+/* The same rule inside batches of two or more queries, where one dense pass
+ * over the index serves a whole group of dense queries (default 0.003). */
+ab_status ab_ctx_set_batch_threshold(ab_ctx *ctx, double tau);
+double ab_ctx_get_batch_threshold(const ab_ctx *ctx);
 /* Dense group width (queries per key-stream pass, 1..64). */
 ab_status ab_ctx_set_dense_group(ab_ctx *ctx, uint32_t width);
 /* Work-group size of the dense kernel (64, 128, 256, 512 or 1024 threads). */
@@ -235,6 +239,33 @@ ab_status ab_query_features(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t 
 /* Same features from host keys, computed on the host. */
 ab_status ab_query_features_host(const ab_index *ix, const uint32_t *h_keys, uint32_t n,
                                  float *h_feature, uint64_t *h_m);
+
+/* ---- cost model (FP32 MLP 12 -> 64 -> 64 -> 2, hand-written HIP kernels) --
+ * Predicts log execution time of the dense (output 0) and sparse (output 1)
+ * path from AB_NFEATURES standardized features; the dispatcher takes the
+ * smaller.  Parameters: W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64],
+ * b3[2] (row-major, ReLU after the hidden layers).  The model lives in the
+ * context; AB_PATH_AUTO / AB_PATH_LEARNED use it when loaded and fall back to
+ * the threshold rule otherwise.  No reference counterpart: the reference runs
+ * both paths per query and has no dispatcher. */
+uint32_t ab_mlp_nparams(void);
+ab_status ab_mlp_set(ab_ctx *ctx, const float *params, const float *mean, const float *stdv);
+ab_status ab_mlp_get(ab_ctx *ctx, float *params, float *mean, float *stdv);
+ab_status ab_mlp_clear(ab_ctx *ctx);
+/* Device inference for n host feature rows: h_y gets n*2 predictions, h_sel
+ * (may be NULL) the chosen paths. */
+ab_status ab_mlp_predict(ab_ctx *ctx, const float *h_x, uint32_t n, float *h_y, uint8_t *h_sel);
+/* Loss gradient over n (1..4096) samples at the loaded parameters, computed
+ * by the training kernel in gradient-only mode (loss = mean over samples and
+ * outputs of 0.5*(y_hat - y)^2). */
+ab_status ab_mlp_grad(ab_ctx *ctx, const float *h_x, const float *h_y, uint32_t n, float *h_grad,
+                      float *h_loss);
+/* Adam training on the GPU over n samples (targets n*2 log times), shuffled
+ * minibatches of `batch` (<= 4096) rows for `epochs` epochs.  Starts from the
+ * loaded model, or from a seeded initialization with mean/std taken from the
+ * data.  loss_curve (may be NULL) receives the mean loss of each epoch. */
+ab_status ab_mlp_train(ab_ctx *ctx, const float *h_x, const float *h_y, uint32_t n, uint32_t epochs,
+                       uint32_t batch, float lr, uint64_t seed, float *loss_curve);
 
 /* ---- synthetic data ------------------------------------------------------- */
 typedef struct ab_dataset ab_dataset;

@@ -93,12 +93,12 @@ def test_batch_mixed_paths(c1):
     nk = configs.C1.nkeys
     qk, qo = ab.gen_queries(0xADAB0101, nk, 48, size_dist=1, p1=60.0, p2=1.3, lo=1, hi=3000, key_dist=1,
                             zipf_s=1.0, perm_seed=configs.C1.seed)
-    old = ctx.threshold
-    ctx.set_threshold(0.01)
+    old = ctx.batch_threshold
+    ctx.set_batch_threshold(0.01)
     try:
         bms, ids, counts, paths = ctx.query_batch(qk, qo, ab.AB_PATH_THRESHOLD)
     finally:
-        ctx.set_threshold(old)
+        ctx.set_batch_threshold(old)
     assert (paths == ab.AB_PATH_DENSE).any() and (paths == ab.AB_PATH_SPARSE).any(), paths
     for i in range(48):
         want_bm, want_ids = csr.union(qk[qo[i]:qo[i + 1]])
