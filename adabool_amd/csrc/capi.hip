@@ -479,8 +479,8 @@ extern "C" ab_status ab_ctx_set_dense_group(ab_ctx *c, uint32_t w)
 
 extern "C" ab_status ab_ctx_set_dense_threads(ab_ctx *c, uint32_t t)
 {
-    if (!c || (t != 64 && t != 128 && t != 256 && t != 512 && t != 1024)) {
-        set_error("ab_ctx_set_dense_threads: 64, 128, 256, 512 or 1024");
+    if (!c || (t != 64 && t != 128 && t != 256 && t != 512)) {
+        set_error("ab_ctx_set_dense_threads: 64, 128, 256 or 512");
         return AB_INVALID_ARG;
     }
     c->dense_threads = t;
@@ -572,7 +572,7 @@ extern "C" ab_status ab_ctx_timing(ab_ctx *c, double *ms, uint64_t *n, int reset
 static uint32_t max_group(const ab_ctx *c)
 {
     uint32_t g = c->group;
-    while (g > 1 && dense_lds_bytes(g, c->ix->tile_shift) > 65536) --g;
+    while (g > 1 && dense_lds_bytes(g, c->ix->tile_shift, c->dense_threads) > 65536) --g;
     return g;
 }
 

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "adabool.h"
+#include "layout.h"
 
 namespace ab {
 
@@ -24,15 +25,6 @@ constexpr int kPayloadBytes[kTypes] = {0, 4, 8, 16, 32, 32};
 // bytes emitted unconditionally before the zero terminator applies.
 constexpr int kMinNum[kTypes] = {0, 1, 5, 9, 17, 0};
 
-// Records per key-stream block of the value-tiled layout; a wave64 decodes a
-// block with 16 records per lane (record s*64 + lane in slot s).
-constexpr uint32_t kBlockRecs = 1024;
-constexpr uint32_t kSlots = kBlockRecs / 64;
-
-// seg8 row layout: 8 u32 per key boundary.
-//   [0..5] per-type record prefix (key_segments_), [6] all-type record prefix,
-//   [7] value (nnz) prefix.
-constexpr int kSegStride = 8;
 
 // Block descriptor of the value-tiled key stream.
 struct BlockMeta {

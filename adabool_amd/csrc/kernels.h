@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "layout.h"
+
 namespace ab {
 
 // Device view of a resident index (both layouts), passed to kernels by value.
@@ -50,7 +52,7 @@ void launch_mask_keys(int mode, const uint32_t *keys, const uint64_t *qoff, cons
 void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_t b0, uint32_t B,
                               void *mask, uint32_t *anybm, uint32_t nkeys, hipStream_t s);
 // dynamic LDS bytes of the dense kernel for a group of B queries
-uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift);
+uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift, uint32_t threads);
 hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask,
                               const uint32_t *anybm, const DenseGroup &g, uint32_t *out,
                               uint64_t out_stride, uint64_t out_words, hipStream_t s);

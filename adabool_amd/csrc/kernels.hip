@@ -487,47 +487,130 @@ __device__ __forceinline__ typename MaskOf<MODE>::T probe(const void *__restrict
         return hit ? reinterpret_cast<const unsigned long long *>(m)[k] : 0ull;
 }
 
+// Picks w[j] for a run-time j < N without dynamic register indexing.
+template <int N>
+__device__ __forceinline__ uint32_t pick(const uint32_t (&w)[8], uint32_t j)
+{
+    uint32_t r = w[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) r = (j == (uint32_t)i) ? w[i] : r;
+    return r;
+}
+
+// ORs one matched record into the LDS tile of every query whose bit is set in
+// m.  The record's values are sorted and share one 256-value window, so each
+// run of values in one 32-bit word is folded into a single OR.  The payload
+// is walked word by word (4 bytes per step) to keep one copy of the code.
 template <int MODE, int T>
-__device__ __forceinline__ void dense_emit_block(const DevIndex &ix, uint32_t rec_base,
-                                                 const typename MaskOf<MODE>::T (&mk)[16], uint32_t lane,
-                                                 uint32_t *tiles, uint32_t tw, uint32_t tile_word0)
+__device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], typename MaskOf<MODE>::T m,
+                                           uint32_t *tiles, uint32_t tw, uint32_t tile_word0)
+{
+    using M = typename MaskOf<MODE>::T;
+    auto flush = [&](uint32_t gw, uint32_t bits) {
+        const uint32_t lw = gw - tile_word0;
+        if constexpr (MODE == kModeBitmap) {
+            atomicOr(&tiles[lw], bits);
+        } else {
+            M mm = m;
+            while (mm) {
+                const uint32_t b = (uint32_t)__builtin_ctzll((unsigned long long)mm);
+                mm &= mm - 1;
+                atomicOr(&tiles[b * tw + lw], bits);
+            }
+        }
+    };
+    if constexpr (T == AB_BITSET) {
+        const uint32_t base = h >> 5;
+#pragma unroll 1
+        for (uint32_t j = 0; j < 8; ++j) {
+            const uint32_t x = pick<8>(w, j);
+            if (x) flush(base + j, x);
+        }
+    } else if constexpr (T == AB_ARRAY1) {
+        flush(h >> 5, 1u << (h & 31));
+    } else {
+        constexpr int P = TypeTraits<T>::P, kMin = TypeTraits<T>::kMin;
+        const uint32_t hi = h & 0xffffff00u;
+        uint32_t cw = h >> 5, bits = 1u << (h & 31);
+        bool stop = false;
+#pragma unroll 1
+        for (uint32_t j = 0; j < (uint32_t)(P / 4) && !stop; ++j) {
+            const uint32_t word = pick<P / 4>(w, j);
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                const uint32_t b = (word >> (8 * k)) & 0xffu;
+                if (!stop && (j * 4 + k >= (uint32_t)kMin) && b == 0) stop = true;
+                if (!stop) {
+                    const uint32_t v = hi | b;
+                    if ((v >> 5) != cw) {
+                        flush(cw, bits);
+                        cw = v >> 5;
+                        bits = 0;
+                    }
+                    bits |= 1u << (v & 31);
+                }
+            }
+        }
+        flush(cw, bits);
+    }
+}
+
+// One 1024-record block of type T whose absolute keys are staged in kb
+// (record s*64 + lane at kb[s*64 + lane]).  Slots go four at a time: the four
+// probes, then the four header/payload loads, are issued before any emit.
+template <int MODE, int T>
+__device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *kb, uint32_t nv, uint32_t rec_base,
+                                            const void *__restrict__ mask, const uint32_t *__restrict__ anybm,
+                                            uint32_t lane, uint32_t *tiles, uint32_t tw, uint32_t tile_word0)
 {
     using M = typename MaskOf<MODE>::T;
     const uint32_t *__restrict__ hdr = ix.vhdr[T];
     const uint8_t *__restrict__ pay = ix.vpay[T];
+#pragma unroll 1
+    for (uint32_t s0 = 0; s0 < kSlots; s0 += 4) {
+        M m[4];
 #pragma unroll
-    for (int s = 0; s < 16; ++s) {
-        const M m = mk[s];
-        if (m) {
-            const uint32_t idx = rec_base + (uint32_t)s * 64 + lane;
-            const uint32_t h = hdr[idx];
-            uint32_t w[8];
-            load_payload<T>(pay, idx, w);
-            emit_record<T>(h, w, [&](uint32_t gw, uint32_t bits) {
-                const uint32_t lw = gw - tile_word0;
-                if constexpr (MODE == kModeBitmap) {
-                    atomicOr(&tiles[lw], bits);
-                } else {
-                    M mm = m;
-                    while (mm) {
-                        const uint32_t b = (uint32_t)__builtin_ctzll((unsigned long long)mm);
-                        mm &= mm - 1;
-                        atomicOr(&tiles[b * tw + lw], bits);
-                    }
-                }
-            });
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t r = (s0 + j) * 64 + lane;
+            m[j] = r < nv ? probe<MODE>(mask, anybm, kb[r]) : (M)0;
+        }
+        if (!(m[0] | m[1] | m[2] | m[3])) continue;
+        uint32_t h[4], w[4][8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            h[j] = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) w[j][i] = 0;
+            if (m[j]) {
+                const uint32_t idx = rec_base + (s0 + j) * 64 + lane;
+                h[j] = hdr[idx];
+                load_payload<T>(pay, idx, w[j]);
+            }
+        }
+#pragma unroll 1
+        for (uint32_t j = 0; j < 4; ++j) {
+            const M mj = j == 0 ? m[0] : (j == 1 ? m[1] : (j == 2 ? m[2] : m[3]));
+            if (!mj) continue;
+            const uint32_t hj = j == 0 ? h[0] : (j == 1 ? h[1] : (j == 2 ? h[2] : h[3]));
+            uint32_t wj[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                wj[i] = j == 0 ? w[0][i] : (j == 1 ? w[1][i] : (j == 2 ? w[2][i] : w[3][i]));
+            dense_emit<MODE, T>(hj, wj, mj, tiles, tw, tile_word0);
         }
     }
 }
 
+// Dynamic LDS: per-wave key staging (1024 u32 per wave) then B query tiles.
 template <int MODE, int NT>
 __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__restrict__ mask,
                                                     const uint32_t *__restrict__ anybm, DenseGroup g,
                                                     uint32_t *__restrict__ out, uint64_t out_stride,
                                                     uint64_t out_words)
 {
-    using M = typename MaskOf<MODE>::T;
-    extern __shared__ uint32_t tiles[];
+    constexpr uint32_t NW = NT / 64;
+    extern __shared__ uint32_t lds[];
+    uint32_t *tiles = lds + NW * kBlockRecs;
     const uint32_t tile = blockIdx.x;
     const uint32_t tw = 1u << (ix.tile_shift - 5);
     const uint32_t tile_word0 = tile * tw;
@@ -537,70 +620,82 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
 
     const uint32_t lane = lane_id();
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    constexpr uint32_t NW = NT / 64;
+    uint32_t *kb = lds + wave * kBlockRecs;
     const uint32_t b0 = ix.tile_blk[tile], b1 = ix.tile_blk[tile + 1];
     for (uint32_t b = b0 + wave; b < b1; b += NW) {
         const uint4 mt = ix.vblk[b];
         const uint32_t type = mt.z & 15u, wl = (mt.z >> 4) & 3u, nv = mt.z >> 8;
         const uint8_t *kp = ix.vkeys + mt.y;
-        uint32_t key[16];
+        // decode the block's keys (record s*64 + lane in slot s) into the wave's staging area
         if (wl == 0) {
             const uint4 v = *reinterpret_cast<const uint4 *>(kp + lane * 16u);
             const uint32_t ww[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int s = 0; s < 16; ++s) key[s] = (ww[s >> 2] >> (8 * (s & 3))) & 0xffu;
+            for (int s = 0; s < 16; ++s) kb[s * 64 + lane] = mt.x + ((ww[s >> 2] >> (8 * (s & 3))) & 0xffu);
         } else if (wl == 1) {
             const uint4 *p = reinterpret_cast<const uint4 *>(kp + lane * 32u);
             const uint4 v0 = p[0], v1 = p[1];
             const uint32_t ww[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
 #pragma unroll
-            for (int s = 0; s < 16; ++s) key[s] = (ww[s >> 1] >> (16 * (s & 1))) & 0xffffu;
+            for (int s = 0; s < 16; ++s) kb[s * 64 + lane] = mt.x + ((ww[s >> 1] >> (16 * (s & 1))) & 0xffffu);
         } else {
             const uint4 *p = reinterpret_cast<const uint4 *>(kp + lane * 64u);
             const uint4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
             const uint32_t ww[16] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w,
                                      v2.x, v2.y, v2.z, v2.w, v3.x, v3.y, v3.z, v3.w};
 #pragma unroll
-            for (int s = 0; s < 16; ++s) key[s] = ww[s];
+            for (int s = 0; s < 16; ++s) kb[s * 64 + lane] = mt.x + ww[s];
         }
-        M mk[16];
-#pragma unroll
-        for (int s = 0; s < 16; ++s) {
-            const uint32_t r = (uint32_t)s * 64 + lane;
-            mk[s] = r < nv ? probe<MODE>(mask, anybm, mt.x + key[s]) : (M)0;
-        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         switch (type) {
-        case AB_ARRAY1: dense_emit_block<MODE, AB_ARRAY1>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY4: dense_emit_block<MODE, AB_ARRAY4>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY8: dense_emit_block<MODE, AB_ARRAY8>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY16: dense_emit_block<MODE, AB_ARRAY16>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY32: dense_emit_block<MODE, AB_ARRAY32>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
-        default: dense_emit_block<MODE, AB_BITSET>(ix, mt.w, mk, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY1: dense_block<MODE, AB_ARRAY1>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY4: dense_block<MODE, AB_ARRAY4>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY8: dense_block<MODE, AB_ARRAY8>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY16: dense_block<MODE, AB_ARRAY16>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY32: dense_block<MODE, AB_ARRAY32>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
+        default: dense_block<MODE, AB_BITSET>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
         }
+        // the staging area is rewritten by the next block of this wave
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
     // write-once: every word of the tile, for every query of the group
     const uint64_t nwq = out_words > tile_word0 ? ((out_words - tile_word0) < tw ? out_words - tile_word0 : tw) : 0;
-    for (uint32_t b = 0; b < g.B; ++b) {
-        uint32_t *dst = out + (uint64_t)g.q[b] * out_stride + tile_word0;
-        for (uint32_t i = threadIdx.x; i < nwq; i += NT) dst[i] = tiles[b * tw + i];
+    if ((out_stride & 3) == 0 && (nwq & 3) == 0) {
+        const uint32_t n4 = (uint32_t)(nwq >> 2), tw4 = tw >> 2;
+        const uint4 *src = reinterpret_cast<const uint4 *>(tiles);
+        for (uint32_t i = threadIdx.x; i < g.B * n4; i += NT) {
+            const uint32_t b = i / n4, k = i - b * n4;
+            reinterpret_cast<uint4 *>(out + (uint64_t)g.q[b] * out_stride + tile_word0)[k] = src[b * tw4 + k];
+        }
+    } else {
+        for (uint32_t b = 0; b < g.B; ++b) {
+            uint32_t *dst = out + (uint64_t)g.q[b] * out_stride + tile_word0;
+            for (uint32_t i = threadIdx.x; i < nwq; i += NT) dst[i] = tiles[b * tw + i];
+        }
     }
 }
 
-uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift) { return B * (1u << (tile_shift - 5)) * 4u; }
+uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift, uint32_t threads)
+{
+    return (threads / 64) * kBlockRecs * 4u + B * (1u << (tile_shift - 5)) * 4u;
+}
 
 template <int MODE>
 static hipError_t launch_dense_mode(uint32_t threads, const DevIndex &ix, const void *mask, const uint32_t *anybm,
                                     const DenseGroup &g, uint32_t *out, uint64_t out_stride, uint64_t out_words,
                                     hipStream_t s)
 {
-    const uint32_t lds = dense_lds_bytes(g.B, ix.tile_shift);
+    const uint32_t lds = dense_lds_bytes(g.B, ix.tile_shift, threads);
     dim3 grid(ix.ntiles);
     switch (threads) {
     case 64: hipLaunchKernelGGL((k_dense_tiles<MODE, 64>), grid, dim3(64), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
     case 128: hipLaunchKernelGGL((k_dense_tiles<MODE, 128>), grid, dim3(128), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
     case 512: hipLaunchKernelGGL((k_dense_tiles<MODE, 512>), grid, dim3(512), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
-    case 1024: hipLaunchKernelGGL((k_dense_tiles<MODE, 1024>), grid, dim3(1024), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
     default: hipLaunchKernelGGL((k_dense_tiles<MODE, 256>), grid, dim3(256), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
     }
     return hipGetLastError();
@@ -644,15 +739,17 @@ void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32
     hipLaunchKernelGGL(k_popcount, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt);
 }
 
-// Each wave expands 512 consecutive words: per 64-word step, a wave prefix of
-// the words' popcounts places every lane's ids in an LDS staging area that the
-// wave then stores to the id list with coalesced writes.
+// Each wave expands 512 consecutive words (8 per lane, kept in registers
+// from the counting pass).  Per 64-word step a wave prefix sum of popcounts
+// gives every lane its output position; the lane then stores its ids
+// directly: single dwords up to a 16-byte boundary, then 4 ids per 128-bit
+// store, then the tail.  ALIGNED: ids + q * ids_stride is 16-byte aligned.
+template <bool ALIGNED>
 __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
                                                    uint32_t nchunk, const uint32_t *__restrict__ ccnt,
                                                    uint32_t *__restrict__ ids, uint64_t ids_stride,
                                                    uint32_t *__restrict__ counts)
 {
-    __shared__ uint32_t stage[4][64 * 32];
     __shared__ uint32_t wcnt[4];
     __shared__ uint32_t sred[4];
     const uint32_t q = blockIdx.y, c = blockIdx.x;
@@ -662,46 +759,52 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
     for (uint32_t i = threadIdx.x; i < c; i += 256) pre += cc[i];
     pre = wave_sum(pre);
     if (lane == 0) sred[wave] = pre;
-    __syncthreads();
-    const uint64_t prefix = (uint64_t)sred[0] + sred[1] + sred[2] + sred[3];
-    if (c == nchunk - 1 && threadIdx.x == 0 && counts) counts[q] = (uint32_t)(prefix + cc[c]);
-    if (!ids) return;
     const uint32_t *src = bm + (uint64_t)q * stride;
     const uint64_t ww0 = (uint64_t)c * kCompactWords + (uint64_t)wave * 512;
-    uint32_t mine = 0;
-    for (uint32_t i = 0; i < 8; ++i) {
+    uint32_t x[8], mine = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
         const uint64_t wi = ww0 + i * 64 + lane;
-        if (wi < words) mine += __popc(src[wi]);
+        x[i] = wi < words ? src[wi] : 0u;
+        mine += __popc(x[i]);
     }
     mine = wave_sum(mine);
     if (lane == 0) wcnt[wave] = mine;
     __syncthreads();
+    const uint64_t prefix = (uint64_t)sred[0] + sred[1] + sred[2] + sred[3];
+    if (c == nchunk - 1 && threadIdx.x == 0 && counts) counts[q] = (uint32_t)(prefix + cc[c]);
+    if (!ids) return;
     uint64_t off = prefix;
     for (uint32_t w = 0; w < wave; ++w) off += wcnt[w];
     uint32_t *dst = ids + (uint64_t)q * ids_stride;
-    uint32_t *stg = stage[wave];
-    for (uint32_t i = 0; i < 8; ++i) {
-        const uint64_t wi = ww0 + i * 64 + lane;
-        uint32_t x = wi < words ? src[wi] : 0u;
-        const uint32_t cnt = __popc(x);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint32_t v = x[i];
+        const uint32_t cnt = __popc(v);
         const uint32_t incl = wave_incl_scan(cnt);
         const uint32_t tot = __shfl(incl, 63, 64);
-        if (tot == 0) continue;
-        uint32_t p = incl - cnt;
-        const uint32_t base = (uint32_t)(wi * 32);
-        while (x) {
-            const uint32_t b = __builtin_ctz(x);
-            stg[p++] = base + b;
-            x &= x - 1;
+        uint64_t pos = off + incl - cnt;
+        const uint32_t base = (uint32_t)((ww0 + i * 64 + lane) * 32);
+        if constexpr (ALIGNED) {
+            while (v && (pos & 3)) {
+                dst[pos++] = base + __builtin_ctz(v);
+                v &= v - 1;
+            }
+            while (__popc(v) >= 4) {
+                uint4 o;
+                o.x = base + __builtin_ctz(v); v &= v - 1;
+                o.y = base + __builtin_ctz(v); v &= v - 1;
+                o.z = base + __builtin_ctz(v); v &= v - 1;
+                o.w = base + __builtin_ctz(v); v &= v - 1;
+                *reinterpret_cast<uint4 *>(dst + pos) = o;
+                pos += 4;
+            }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (uint32_t j = lane; j < tot; j += 64) dst[off + j] = stg[j];
+        while (v) {
+            dst[pos++] = base + __builtin_ctz(v);
+            v &= v - 1;
+        }
         off += tot;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -710,8 +813,13 @@ void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint3
 {
     if (!nq || !words) return;
     uint32_t nchunk = (uint32_t)((words + kCompactWords - 1) / kCompactWords);
-    hipLaunchKernelGGL(k_write_ids, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt, ids,
-                       ids_stride, counts);
+    const bool aligned = ((uintptr_t)ids % 16 == 0) && (ids_stride % 4 == 0);
+    if (aligned)
+        hipLaunchKernelGGL(k_write_ids<true>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt, ids,
+                           ids_stride, counts);
+    else
+        hipLaunchKernelGGL(k_write_ids<false>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt,
+                           ids, ids_stride, counts);
 }
 
 // ---------------------------------------------------------------------------

@@ -145,7 +145,7 @@ ab_status ab_ctx_set_batch_threshold(ab_ctx *ctx, double tau);
 double ab_ctx_get_batch_threshold(const ab_ctx *ctx);
 /* Dense group width (queries per key-stream pass, 1..64). */
 ab_status ab_ctx_set_dense_group(ab_ctx *ctx, uint32_t width);
-/* Work-group size of the dense kernel (64, 128, 256, 512 or 1024 threads). */
+/* Work-group size of the dense kernel (64, 128, 256 or 512 threads). */
 ab_status ab_ctx_set_dense_threads(ab_ctx *ctx, uint32_t threads);
 /* Records per work item of the sparse kernel (power of two, 256..8192). */
 ab_status ab_ctx_set_sparse_chunk(ab_ctx *ctx, uint32_t items);

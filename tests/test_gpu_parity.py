@@ -140,7 +140,7 @@ def test_batch_sparse_chunks(c1, chunk):
         assert np.array_equal(ids[i], want_ids)
 
 
-@pytest.mark.parametrize("threads", [64, 128, 512, 1024])
+@pytest.mark.parametrize("threads", [64, 128, 512])
 def test_dense_threads(c1, threads):
     _, _, ix, ctx, csr = c1
     qk, qo = configs.make_queries(configs.C1, configs.C1_Q, 3)
