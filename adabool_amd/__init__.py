@@ -101,6 +101,7 @@ _SIGS = {
     "ab_ctx_set_dense_group": (C.c_int, [_VP, C.c_uint32]),
     "ab_ctx_set_dense_threads": (C.c_int, [_VP, C.c_uint32]),
     "ab_ctx_set_sparse_chunk": (C.c_int, [_VP, C.c_uint32]),
+    "ab_ctx_set_dense_columns": (C.c_int, [_VP, C.c_int]),
     "ab_ctx_set_timing": (C.c_int, [_VP, C.c_int]),
     "ab_ctx_timing": (C.c_int, [_VP, _VP, _VP, C.c_int]),
     "ab_query": (C.c_int, [_VP, _VP, C.c_uint32, C.c_int, _VP, _VP, _VP, C.POINTER(C.c_int)]),
@@ -407,6 +408,9 @@ class Context:
 
     def set_dense_threads(self, t: int):
         _check(_lib.ab_ctx_set_dense_threads(self._h, t))
+
+    def set_dense_columns(self, on: bool):
+        _check(_lib.ab_ctx_set_dense_columns(self._h, 1 if on else 0))
 
     def set_sparse_chunk(self, n: int):
         _check(_lib.ab_ctx_set_sparse_chunk(self._h, n))

@@ -487,6 +487,16 @@ extern "C" ab_status ab_ctx_set_dense_threads(ab_ctx *c, uint32_t t)
     return AB_OK;
 }
 
+extern "C" ab_status ab_ctx_set_dense_columns(ab_ctx *c, int on)
+{
+    if (!c) {
+        set_error("ab_ctx_set_dense_columns: NULL context");
+        return AB_INVALID_ARG;
+    }
+    c->dense_columns = on != 0;
+    return AB_OK;
+}
+
 extern "C" ab_status ab_ctx_set_sparse_chunk(ab_ctx *c, uint32_t items)
 {
     if (!c || items < 256 || items > 8192 || (items & (items - 1))) {
@@ -568,12 +578,22 @@ extern "C" ab_status ab_ctx_timing(ab_ctx *c, double *ms, uint64_t *n, int reset
 // ---------------------------------------------------------------------------
 // orchestration
 // ---------------------------------------------------------------------------
-// Largest dense group whose LDS tiles fit the 64 KiB dynamic-LDS budget.
+static int group_mode(uint32_t B) { return B == 1 ? kModeBitmap : (B <= 32 ? kModeMask32 : kModeMask64); }
+
+constexpr uint32_t kLdsBudget = 65536;
+
+// Largest dense group whose LDS (key staging and row tiles) fits the dynamic
+// LDS budget; the column masks are added when they fit too.
 static uint32_t max_group(const ab_ctx *c)
 {
     uint32_t g = c->group;
-    while (g > 1 && dense_lds_bytes(g, c->ix->tile_shift, c->dense_threads) > 65536) --g;
+    while (g > 1 && dense_lds_bytes(group_mode(g), g, c->ix->tile_shift, c->dense_threads, false) > kLdsBudget) --g;
     return g;
+}
+
+static bool col_fits(const ab_ctx *c, int mode, uint32_t B)
+{
+    return c->dense_columns && dense_lds_bytes(mode, B, c->ix->tile_shift, c->dense_threads, true) <= kLdsBudget;
 }
 
 // Waits until the pinned staging area is no longer read by an earlier copy.
@@ -709,7 +729,7 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
             DenseGroup g;
             g.B = (uint32_t)std::min<size_t>(G, dq.size() - g0);
             for (uint32_t b = 0; b < g.B; ++b) g.q[b] = dq[g0 + b];
-            int mode = g.B == 1 ? kModeBitmap : (g.B <= 32 ? kModeMask32 : kModeMask64);
+            const int mode = group_mode(g.B);
             void *mask = c->qbm.p;
             if (mode != kModeBitmap) {
                 if (!c->mask.p) {
@@ -725,7 +745,7 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
             {
                 TimedSpan ts(c, AB_K_DENSE_TILES);
                 HIP_TRY(launch_dense_tiles(mode, c->dense_threads, d, mask, (const uint32_t *)c->qbm.p, g, bm, out_stride,
-                                           out_words, s));
+                                           out_words, col_fits(c, mode, g.B), s));
             }
             TimedSpan ts(c, AB_K_DENSE_MASK);
             launch_mask_keys(mode, a.d_keys, qoff_d, g, mask, (uint32_t *)c->qbm.p, ix->nkeys, 1, s);
@@ -836,7 +856,7 @@ extern "C" ab_status ab_list_union_dense(ab_ctx *c, uint64_t len_bytes, const ui
     g.q[0] = 0;
     const uint64_t w = ix->stride / 4;
     HIP_TRY(launch_dense_tiles(kModeBitmap, c->dense_threads, ix->dev, c->qbm.p, (const uint32_t *)c->qbm.p, g, d_res, w,
-                               w, c->stream));
+                               w, false, c->stream));
     HIP_TRY(hipMemsetAsync(c->qbm.p, 0, qbytes, c->stream));
     return AB_OK;
 }
@@ -857,7 +877,8 @@ extern "C" ab_status ab_list_union_dense_batch(ab_ctx *c, uint32_t B, uint64_t l
         AB_TRY(c->mask.ensure((size_t)ix->nkeys * 8 + 64));
         HIP_TRY(hipMemsetAsync(c->mask.p, 0, c->mask.cap, c->stream));
     }
-    const uint32_t G = std::max<uint32_t>(2, std::min<uint32_t>(max_group(c), 64));
+    uint32_t G = 64;
+    while (G > 1 && dense_lds_bytes(kModeMask64, G, ix->tile_shift, c->dense_threads, false) > kLdsBudget) --G;
     const uint64_t out_words = std::min<uint64_t>(res_len_u32, ix->stride / 4);
     for (uint32_t b0 = 0; b0 < B; b0 += G) {
         DenseGroup g;
@@ -866,7 +887,7 @@ extern "C" ab_status ab_list_union_dense_batch(ab_ctx *c, uint32_t B, uint64_t l
         launch_mask_from_bitmaps((const uint32_t *)d_bitmaps, len_bytes / 4, b0, g.B, c->mask.p,
                                  (uint32_t *)c->qbm.p, ix->nkeys, c->stream);
         HIP_TRY(launch_dense_tiles(kModeMask64, c->dense_threads, ix->dev, c->mask.p, (const uint32_t *)c->qbm.p, g,
-                                   d_res, res_len_u32, out_words, c->stream));
+                                   d_res, res_len_u32, out_words, col_fits(c, kModeMask64, g.B), c->stream));
     }
     // both filters back to all-zero for the key-list paths
     HIP_TRY(hipMemsetAsync(c->mask.p, 0, (size_t)ix->nkeys * 8, c->stream));

@@ -74,6 +74,7 @@ struct ab_ctx {
     double tau_batch = 0.003;   // threshold rule inside batches (dense passes are shared)
     uint32_t group = 64;
     uint32_t dense_threads = 256;
+    bool dense_columns = true;  // per-value query masks in dense groups when LDS allows
     uint32_t sparse_chunk = 4096;
     // device scratch
     ab::DevBuf qoff, sel, kq, cnt, koff, parts, feat, M, ccnt, bm, qlist, keys, qbm, mask;

@@ -51,11 +51,12 @@ void launch_mask_keys(int mode, const uint32_t *keys, const uint64_t *qoff, cons
 // Writes mask[k] (u64) and anybm for every k < nkeys from B query bitmaps.
 void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_t b0, uint32_t B,
                               void *mask, uint32_t *anybm, uint32_t nkeys, hipStream_t s);
-// dynamic LDS bytes of the dense kernel for a group of B queries
-uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift, uint32_t threads);
+// Dynamic LDS bytes of the dense kernel for a group of B queries in a mode,
+// with or without the per-value column masks (group modes only).
+uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t threads, bool use_col);
 hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask,
                               const uint32_t *anybm, const DenseGroup &g, uint32_t *out,
-                              uint64_t out_stride, uint64_t out_words, hipStream_t s);
+                              uint64_t out_stride, uint64_t out_words, bool use_col, hipStream_t s);
 constexpr uint32_t kCompactWords = 2048;
 void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
                      uint32_t *ccnt, hipStream_t s);

@@ -497,13 +497,19 @@ __device__ __forceinline__ uint32_t pick(const uint32_t (&w)[8], uint32_t j)
     return r;
 }
 
-// ORs one matched record into the LDS tile of every query whose bit is set in
-// m.  The record's values are sorted and share one 256-value window, so each
-// run of values in one 32-bit word is folded into a single OR.  The payload
-// is walked word by word (4 bytes per step) to keep one copy of the code.
+// ORs one matched record into the LDS tiles of every query whose bit is set in
+// m.  Two accumulators: row tiles (one bit row per query; a record costs one
+// OR per touched word per query) and, for query groups, a column mask per
+// value (the set of queries containing that value; a record costs one OR per
+// value).  Each record takes the cheaper one; the write-out transposes the
+// columns into the rows with wave ballots.  The record's values are sorted
+// and share one 256-value window, so runs of values in one word fold into a
+// single row OR.  The payload is walked 4 bytes per step to keep one copy of
+// the code.
 template <int MODE, int T>
 __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], typename MaskOf<MODE>::T m,
-                                           uint32_t *tiles, uint32_t tw, uint32_t tile_word0)
+                                           uint32_t *tiles, uint32_t tw, uint32_t tile_word0,
+                                           typename MaskOf<MODE>::T *col, uint32_t vmask)
 {
     using M = typename MaskOf<MODE>::T;
     auto flush = [&](uint32_t gw, uint32_t bits) {
@@ -519,19 +525,53 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
             }
         }
     };
+    auto put = [&](uint32_t v) { atomicOr(&col[v & vmask], m); };
+    uint32_t pc = 1;
+    if constexpr (MODE != kModeBitmap) pc = col ? (uint32_t)__popcll((unsigned long long)m) : 0u;
     if constexpr (T == AB_BITSET) {
         const uint32_t base = h >> 5;
+        bool use_col = false;
+        if constexpr (MODE != kModeBitmap) {
+            uint32_t nvals = 0, nw = 0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                nvals += __popc(w[j]);
+                nw += w[j] != 0;
+            }
+            use_col = nvals <= nw * pc;
+        }
 #pragma unroll 1
         for (uint32_t j = 0; j < 8; ++j) {
-            const uint32_t x = pick<8>(w, j);
-            if (x) flush(base + j, x);
+            uint32_t x = pick<8>(w, j);
+            if (!x) continue;
+            if constexpr (MODE != kModeBitmap) {
+                if (use_col) {
+                    while (x) {
+                        put((h & 0xffffff00u) + j * 32 + (uint32_t)__builtin_ctz(x));
+                        x &= x - 1;
+                    }
+                    continue;
+                }
+            }
+            flush(base + j, x);
         }
     } else if constexpr (T == AB_ARRAY1) {
+        if constexpr (MODE != kModeBitmap) {
+            if (pc >= 2) {
+                put(h);
+                return;
+            }
+        }
         flush(h >> 5, 1u << (h & 31));
     } else {
         constexpr int P = TypeTraits<T>::P, kMin = TypeTraits<T>::kMin;
+        // column emission wins once a word run is shared by enough queries
+        constexpr uint32_t kColMin = P <= 8 ? 2 : (P == 16 ? 3 : 5);
+        bool use_col = false;
+        if constexpr (MODE != kModeBitmap) use_col = pc >= kColMin;
         const uint32_t hi = h & 0xffffff00u;
         uint32_t cw = h >> 5, bits = 1u << (h & 31);
+        if (use_col) put(h);
         bool stop = false;
 #pragma unroll 1
         for (uint32_t j = 0; j < (uint32_t)(P / 4) && !stop; ++j) {
@@ -542,16 +582,20 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
                 if (!stop && (j * 4 + k >= (uint32_t)kMin) && b == 0) stop = true;
                 if (!stop) {
                     const uint32_t v = hi | b;
-                    if ((v >> 5) != cw) {
-                        flush(cw, bits);
-                        cw = v >> 5;
-                        bits = 0;
+                    if (use_col) {
+                        put(v);
+                    } else {
+                        if ((v >> 5) != cw) {
+                            flush(cw, bits);
+                            cw = v >> 5;
+                            bits = 0;
+                        }
+                        bits |= 1u << (v & 31);
                     }
-                    bits |= 1u << (v & 31);
                 }
             }
         }
-        flush(cw, bits);
+        if (!use_col) flush(cw, bits);
     }
 }
 
@@ -561,7 +605,8 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
 template <int MODE, int T>
 __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *kb, uint32_t nv, uint32_t rec_base,
                                             const void *__restrict__ mask, const uint32_t *__restrict__ anybm,
-                                            uint32_t lane, uint32_t *tiles, uint32_t tw, uint32_t tile_word0)
+                                            uint32_t lane, uint32_t *tiles, uint32_t tw, uint32_t tile_word0,
+                                            typename MaskOf<MODE>::T *col, uint32_t vmask)
 {
     using M = typename MaskOf<MODE>::T;
     const uint32_t *__restrict__ hdr = ix.vhdr[T];
@@ -596,26 +641,32 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
 #pragma unroll
             for (int i = 0; i < 8; ++i)
                 wj[i] = j == 0 ? w[0][i] : (j == 1 ? w[1][i] : (j == 2 ? w[2][i] : w[3][i]));
-            dense_emit<MODE, T>(hj, wj, mj, tiles, tw, tile_word0);
+            dense_emit<MODE, T>(hj, wj, mj, tiles, tw, tile_word0, col, vmask);
         }
     }
 }
 
-// Dynamic LDS: per-wave key staging (1024 u32 per wave) then B query tiles.
+// Dynamic LDS: per-wave key staging (1024 u32 per wave), B query row tiles,
+// and for query groups one column mask per value of the tile.
 template <int MODE, int NT>
 __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__restrict__ mask,
                                                     const uint32_t *__restrict__ anybm, DenseGroup g,
                                                     uint32_t *__restrict__ out, uint64_t out_stride,
-                                                    uint64_t out_words)
+                                                    uint64_t out_words, int use_col)
 {
+    using M = typename MaskOf<MODE>::T;
     constexpr uint32_t NW = NT / 64;
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     uint32_t *tiles = lds + NW * kBlockRecs;
     const uint32_t tile = blockIdx.x;
-    const uint32_t tw = 1u << (ix.tile_shift - 5);
+    const uint32_t tv = 1u << ix.tile_shift, vmask = tv - 1;
+    const uint32_t tw = tv >> 5;
     const uint32_t tile_word0 = tile * tw;
     const uint32_t ntw = g.B * tw;
+    M *col = (MODE != kModeBitmap && use_col) ? reinterpret_cast<M *>(tiles + ntw) : nullptr;
     for (uint32_t i = threadIdx.x; i < ntw; i += NT) tiles[i] = 0;
+    if (col)
+        for (uint32_t i = threadIdx.x; i < tv; i += NT) col[i] = 0;
     __syncthreads();
 
     const uint32_t lane = lane_id();
@@ -650,12 +701,12 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         switch (type) {
-        case AB_ARRAY1: dense_block<MODE, AB_ARRAY1>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY4: dense_block<MODE, AB_ARRAY4>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY8: dense_block<MODE, AB_ARRAY8>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY16: dense_block<MODE, AB_ARRAY16>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
-        case AB_ARRAY32: dense_block<MODE, AB_ARRAY32>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
-        default: dense_block<MODE, AB_BITSET>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0); break;
+        case AB_ARRAY1: dense_block<MODE, AB_ARRAY1>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY4: dense_block<MODE, AB_ARRAY4>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY8: dense_block<MODE, AB_ARRAY8>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY16: dense_block<MODE, AB_ARRAY16>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY32: dense_block<MODE, AB_ARRAY32>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        default: dense_block<MODE, AB_BITSET>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
         }
         // the staging area is rewritten by the next block of this wave
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -663,6 +714,24 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
+    if (col) {
+        // columns -> rows: lane l holds the query set of value 64*gi + l; the
+        // ballot of bit b over the wave is query b's 64 bits of those values
+        for (uint32_t gi = wave; gi < (tv >> 6); gi += NW) {
+            const unsigned long long qv = (unsigned long long)col[gi * 64 + lane];
+            if (__ballot(qv != 0) == 0) continue;
+            unsigned long long mine = 0;
+            for (uint32_t b = 0; b < g.B; ++b) {
+                const unsigned long long bal = __ballot((qv >> b) & 1ull);
+                if (lane == b) mine = bal;
+            }
+            if (lane < g.B) {
+                tiles[lane * tw + 2 * gi] |= (uint32_t)mine;
+                tiles[lane * tw + 2 * gi + 1] |= (uint32_t)(mine >> 32);
+            }
+        }
+        __syncthreads();
+    }
     // write-once: every word of the tile, for every query of the group
     const uint64_t nwq = out_words > tile_word0 ? ((out_words - tile_word0) < tw ? out_words - tile_word0 : tw) : 0;
     if ((out_stride & 3) == 0 && (nwq & 3) == 0) {
@@ -680,37 +749,40 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
     }
 }
 
-uint32_t dense_lds_bytes(uint32_t B, uint32_t tile_shift, uint32_t threads)
+uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t threads, bool use_col)
 {
-    return (threads / 64) * kBlockRecs * 4u + B * (1u << (tile_shift - 5)) * 4u;
+    const uint32_t col =
+        (mode == kModeBitmap || !use_col) ? 0u : (1u << tile_shift) * (mode == kModeMask32 ? 4u : 8u);
+    return (threads / 64) * kBlockRecs * 4u + B * (1u << (tile_shift - 5)) * 4u + col;
 }
 
 template <int MODE>
 static hipError_t launch_dense_mode(uint32_t threads, const DevIndex &ix, const void *mask, const uint32_t *anybm,
                                     const DenseGroup &g, uint32_t *out, uint64_t out_stride, uint64_t out_words,
-                                    hipStream_t s)
+                                    bool use_col, hipStream_t s)
 {
-    const uint32_t lds = dense_lds_bytes(g.B, ix.tile_shift, threads);
+    const uint32_t lds = dense_lds_bytes(MODE, g.B, ix.tile_shift, threads, use_col);
+    const int uc = use_col ? 1 : 0;
     dim3 grid(ix.ntiles);
     switch (threads) {
-    case 64: hipLaunchKernelGGL((k_dense_tiles<MODE, 64>), grid, dim3(64), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
-    case 128: hipLaunchKernelGGL((k_dense_tiles<MODE, 128>), grid, dim3(128), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
-    case 512: hipLaunchKernelGGL((k_dense_tiles<MODE, 512>), grid, dim3(512), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
-    default: hipLaunchKernelGGL((k_dense_tiles<MODE, 256>), grid, dim3(256), lds, s, ix, mask, anybm, g, out, out_stride, out_words); break;
+    case 64: hipLaunchKernelGGL((k_dense_tiles<MODE, 64>), grid, dim3(64), lds, s, ix, mask, anybm, g, out, out_stride, out_words, uc); break;
+    case 128: hipLaunchKernelGGL((k_dense_tiles<MODE, 128>), grid, dim3(128), lds, s, ix, mask, anybm, g, out, out_stride, out_words, uc); break;
+    case 512: hipLaunchKernelGGL((k_dense_tiles<MODE, 512>), grid, dim3(512), lds, s, ix, mask, anybm, g, out, out_stride, out_words, uc); break;
+    default: hipLaunchKernelGGL((k_dense_tiles<MODE, 256>), grid, dim3(256), lds, s, ix, mask, anybm, g, out, out_stride, out_words, uc); break;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask, const uint32_t *anybm,
                               const DenseGroup &g, uint32_t *out, uint64_t out_stride, uint64_t out_words,
-                              hipStream_t s)
+                              bool use_col, hipStream_t s)
 {
     if (!ix.ntiles || !g.B) return hipSuccess;
     if (mode == kModeBitmap)
-        return launch_dense_mode<kModeBitmap>(threads, ix, mask, anybm, g, out, out_stride, out_words, s);
+        return launch_dense_mode<kModeBitmap>(threads, ix, mask, anybm, g, out, out_stride, out_words, false, s);
     if (mode == kModeMask32)
-        return launch_dense_mode<kModeMask32>(threads, ix, mask, anybm, g, out, out_stride, out_words, s);
-    return launch_dense_mode<kModeMask64>(threads, ix, mask, anybm, g, out, out_stride, out_words, s);
+        return launch_dense_mode<kModeMask32>(threads, ix, mask, anybm, g, out, out_stride, out_words, use_col, s);
+    return launch_dense_mode<kModeMask64>(threads, ix, mask, anybm, g, out, out_stride, out_words, use_col, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -741,15 +813,18 @@ void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32
 
 // Each wave expands 512 consecutive words (8 per lane, kept in registers
 // from the counting pass).  Per 64-word step a wave prefix sum of popcounts
-// gives every lane its output position; the lane then stores its ids
-// directly: single dwords up to a 16-byte boundary, then 4 ids per 128-bit
-// store, then the tail.  ALIGNED: ids + q * ids_stride is 16-byte aligned.
+// places every lane's ids in a wave-private LDS staging area, shifted by the
+// output offset modulo 4 so that staging quads and 16-byte output quads line
+// up; the wave then stores whole quads with 128-bit writes (ALIGNED: ids +
+// q * ids_stride is 16-byte aligned) and the partial end quads dword by dword.
+constexpr uint32_t kStage = 64 * 32 + 4;
 template <bool ALIGNED>
 __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
                                                    uint32_t nchunk, const uint32_t *__restrict__ ccnt,
                                                    uint32_t *__restrict__ ids, uint64_t ids_stride,
                                                    uint32_t *__restrict__ counts)
 {
+    __shared__ __attribute__((aligned(16))) uint32_t stage[4][kStage];
     __shared__ uint32_t wcnt[4];
     __shared__ uint32_t sred[4];
     const uint32_t q = blockIdx.y, c = blockIdx.x;
@@ -777,34 +852,46 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
     uint64_t off = prefix;
     for (uint32_t w = 0; w < wave; ++w) off += wcnt[w];
     uint32_t *dst = ids + (uint64_t)q * ids_stride;
-#pragma unroll
+    uint32_t *stg = stage[wave];
+#pragma unroll 1
     for (int i = 0; i < 8; ++i) {
-        uint32_t v = x[i];
+        uint32_t v = x[0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) v = (i == k) ? x[k] : v;
         const uint32_t cnt = __popc(v);
         const uint32_t incl = wave_incl_scan(cnt);
         const uint32_t tot = __shfl(incl, 63, 64);
-        uint64_t pos = off + incl - cnt;
-        const uint32_t base = (uint32_t)((ww0 + i * 64 + lane) * 32);
-        if constexpr (ALIGNED) {
-            while (v && (pos & 3)) {
-                dst[pos++] = base + __builtin_ctz(v);
-                v &= v - 1;
-            }
-            while (__popc(v) >= 4) {
-                uint4 o;
-                o.x = base + __builtin_ctz(v); v &= v - 1;
-                o.y = base + __builtin_ctz(v); v &= v - 1;
-                o.z = base + __builtin_ctz(v); v &= v - 1;
-                o.w = base + __builtin_ctz(v); v &= v - 1;
-                *reinterpret_cast<uint4 *>(dst + pos) = o;
-                pos += 4;
-            }
-        }
+        if (tot == 0) continue;
+        const uint32_t sh = ALIGNED ? (uint32_t)(off & 3) : 0u;
+        uint32_t p = sh + incl - cnt;
+        const uint32_t base = (uint32_t)((ww0 + (uint64_t)i * 64 + lane) * 32);
         while (v) {
-            dst[pos++] = base + __builtin_ctz(v);
+            stg[p++] = base + __builtin_ctz(v);
             v &= v - 1;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if constexpr (ALIGNED) {
+            // staging index j <-> output index off - sh + j
+            uint32_t *d0 = dst + (off - sh);
+            const uint32_t end = sh + tot, nquad = (end + 3) >> 2;
+            for (uint32_t k = lane; k < nquad; k += 64) {
+                const uint32_t j0 = 4 * k;
+                if (j0 >= sh && j0 + 4 <= end) {
+                    *reinterpret_cast<uint4 *>(d0 + j0) = *reinterpret_cast<const uint4 *>(stg + j0);
+                } else {
+                    for (uint32_t j = j0; j < j0 + 4; ++j)
+                        if (j >= sh && j < end) d0[j] = stg[j];
+                }
+            }
+        } else {
+            for (uint32_t j = lane; j < tot; j += 64) dst[off + j] = stg[j];
+        }
         off += tot;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 

@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--tau-batch", type=float, default=None, help="batch threshold")
     ap.add_argument("--dense-threads", type=int, default=None)
     ap.add_argument("--group", type=int, default=None, help="dense group width")
+    ap.add_argument("--no-columns", action="store_true", help="row-only dense groups")
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB copied once by a plain copy kernel (PMC scale)")
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     args = ap.parse_args()
@@ -145,6 +146,8 @@ def main():
         ctx.set_dense_threads(args.dense_threads)
     if args.group:
         ctx.set_dense_group(args.group)
+    if args.no_columns:
+        ctx.set_dense_columns(False)
     if args.calib_copy:
         n = args.calib_copy * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.int32, device="cuda")

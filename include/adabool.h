@@ -147,6 +147,10 @@ double ab_ctx_get_batch_threshold(const ab_ctx *ctx);
 ab_status ab_ctx_set_dense_group(ab_ctx *ctx, uint32_t width);
 /* Work-group size of the dense kernel (64, 128, 256 or 512 threads). */
 ab_status ab_ctx_set_dense_threads(ab_ctx *ctx, uint32_t threads);
+/* Per-value query-set accumulation in dense groups (on by default; used when
+ * it fits the work-group's LDS).  Off: every match is OR-ed into each
+ * matching query's row tile. */
+ab_status ab_ctx_set_dense_columns(ab_ctx *ctx, int on);
 /* Records per work item of the sparse kernel (power of two, 256..8192). */
 ab_status ab_ctx_set_sparse_chunk(ab_ctx *ctx, uint32_t items);
 

@@ -88,6 +88,40 @@ def test_bitset_and_window_edges():
                 _check(ctx, csr, np.array(q, np.uint32), p, f"ts={ts} q={q}")
 
 
+def _edge_filter():
+    nvalues = 3 * 8192 + 77
+    keys, vals = [], []
+    spans = [[0], [255, 256], [256, 257, 300, 511], list(range(4088, 4096)), list(range(8192, 8206)),
+             list(range(512, 542)), list(range(1024, 1224)), list(range(0, 256, 2)) + [nvalues - 1],
+             [nvalues - 1, nvalues - 2, 8191, 8192], list(range(16384, 16640))]
+    for key, vs in enumerate(spans):
+        keys += [key] * len(vs)
+        vals += vs
+    return np.array(keys, np.uint32), np.array(vals, np.uint32), nvalues
+
+
+@pytest.mark.parametrize("ts", [8, 10, 13])
+@pytest.mark.parametrize("columns", [True, False])
+def test_batch_dense_modes_on_edge_filter(ts, columns):
+    k, v, nvalues = _edge_filter()
+    ix = ab.Index(k, v, 12, nvalues, ts)
+    ctx = ab.Context(ix)
+    ctx.set_dense_columns(columns)
+    csr = oracle.CSR(k, v, 12, nvalues)
+    rng = np.random.default_rng(ts)
+    qs = [rng.choice(12, size=rng.integers(0, 8), replace=False).astype(np.uint32) for _ in range(70)]
+    qs[0] = np.arange(12, dtype=np.uint32)
+    qk = np.concatenate(qs)
+    qo = np.concatenate([[0], np.cumsum([q.size for q in qs])]).astype(np.uint64)
+    for group in (2, 32, 64):
+        ctx.set_dense_group(group)
+        bms, ids, counts, paths = ctx.query_batch(qk, qo, ab.AB_PATH_DENSE)
+        for i, q in enumerate(qs):
+            wb, wi = csr.union(q)
+            assert np.array_equal(bms[i], wb), f"ts={ts} columns={columns} group={group} q{i}"
+            assert np.array_equal(ids[i], wi)
+
+
 def test_batch_mixed_paths(c1):
     _, _, ix, ctx, csr = c1
     nk = configs.C1.nkeys
