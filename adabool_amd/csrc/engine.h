@@ -71,9 +71,9 @@ struct ab_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     double tau = 0.05;          // threshold rule for single queries
-    double tau_batch = 0.003;   // threshold rule inside batches (dense passes are shared)
+    double tau_batch = 0.001;   // threshold rule inside batches (dense passes are shared)
     uint32_t group = 64;
-    uint32_t dense_threads = 256;
+    uint32_t dense_threads = 512;
     bool dense_columns = true;  // per-value query masks in dense groups when LDS allows
     uint32_t sparse_chunk = 4096;
     // device scratch

@@ -140,12 +140,12 @@ void *ab_ctx_stream(ab_ctx *ctx);
 ab_status ab_ctx_set_threshold(ab_ctx *ctx, double tau);
 double ab_ctx_get_threshold(const ab_ctx *ctx);
 /* The same rule inside batches of two or more queries, where one dense pass
- * over the index serves a whole group of dense queries (default 0.003). */
+ * over the index serves a whole group of dense queries (default 0.001). */
 ab_status ab_ctx_set_batch_threshold(ab_ctx *ctx, double tau);
 double ab_ctx_get_batch_threshold(const ab_ctx *ctx);
 /* Dense group width (queries per key-stream pass, 1..64). */
 ab_status ab_ctx_set_dense_group(ab_ctx *ctx, uint32_t width);
-/* Work-group size of the dense kernel (64, 128, 256 or 512 threads). */
+/* Work-group size of the dense kernel (64, 128, 256 or 512 threads; default 512). */
 ab_status ab_ctx_set_dense_threads(ab_ctx *ctx, uint32_t threads);
 /* Per-value query-set accumulation in dense groups (on by default; used when
  * it fits the work-group's LDS).  Off: every match is OR-ed into each

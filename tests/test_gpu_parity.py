@@ -174,7 +174,7 @@ def test_batch_sparse_chunks(c1, chunk):
         assert np.array_equal(ids[i], want_ids)
 
 
-@pytest.mark.parametrize("threads", [64, 128, 512])
+@pytest.mark.parametrize("threads", [64, 128, 256])
 def test_dense_threads(c1, threads):
     _, _, ix, ctx, csr = c1
     qk, qo = configs.make_queries(configs.C1, configs.C1_Q, 3)
@@ -183,7 +183,7 @@ def test_dense_threads(c1, threads):
         for i in range(3):
             _check(ctx, csr, qk[qo[i]:qo[i + 1]], ab.AB_PATH_DENSE, f"threads {threads}")
     finally:
-        ctx.set_dense_threads(256)
+        ctx.set_dense_threads(512)
 
 
 def test_reference_api(c1):
