@@ -45,6 +45,7 @@ __all__ = [
 ]
 
 AB_PATH_AUTO, AB_PATH_DENSE, AB_PATH_SPARSE, AB_PATH_THRESHOLD, AB_PATH_LEARNED = range(5)
+MODEL_SINGLE, MODEL_BATCH = 0, 1
 NTYPES = 6
 PAYLOAD_BYTES = (0, 4, 8, 16, 32, 32)
 NFEATURES = 12
@@ -159,12 +160,13 @@ def load_library(path: Optional[str] = None):
 
 _OPTIONAL_SIGS: dict = {
     "ab_mlp_nparams": (C.c_uint32, []),
-    "ab_mlp_set": (C.c_int, [_VP, _VP, _VP, _VP]),
-    "ab_mlp_get": (C.c_int, [_VP, _VP, _VP, _VP]),
-    "ab_mlp_clear": (C.c_int, [_VP]),
-    "ab_mlp_predict": (C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
-    "ab_mlp_grad": (C.c_int, [_VP, _VP, _VP, C.c_uint32, _VP, _VP]),
-    "ab_mlp_train": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _VP]),
+    "ab_mlp_set": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+    "ab_mlp_get": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
+    "ab_mlp_clear": (C.c_int, [_VP, C.c_int]),
+    "ab_mlp_predict": (C.c_int, [_VP, C.c_int, _VP, C.c_uint32, _VP, _VP]),
+    "ab_mlp_grad": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_uint32, _VP, _VP]),
+    "ab_mlp_train": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64,
+                               _VP]),
 }
 
 
@@ -493,56 +495,63 @@ class Context:
         return bm_h, out_ids, counts, paths
 
     # cost model -----------------------------------------------------------
-    def set_model(self, params, mean, std):
+    # slot 0 (MODEL_SINGLE) prices single queries, slot 1 (MODEL_BATCH) a
+    # query's share of a batch
+    def set_model(self, params, mean, std, slot: int = 0):
         p = np.ascontiguousarray(params, dtype=np.float32)
         mu = np.ascontiguousarray(mean, dtype=np.float32)
         sd = np.ascontiguousarray(std, dtype=np.float32)
         if p.size != _lib.ab_mlp_nparams() or mu.size != NFEATURES or sd.size != NFEATURES:
             raise ValueError("cost model shape mismatch")
-        _check(_lib.ab_mlp_set(self._h, _ptr(p), _ptr(mu), _ptr(sd)))
+        _check(_lib.ab_mlp_set(self._h, slot, _ptr(p), _ptr(mu), _ptr(sd)))
 
-    def get_model(self):
+    def get_model(self, slot: int = 0):
         p = np.zeros(_lib.ab_mlp_nparams(), np.float32)
         mu = np.zeros(NFEATURES, np.float32)
         sd = np.zeros(NFEATURES, np.float32)
-        _check(_lib.ab_mlp_get(self._h, _ptr(p), _ptr(mu), _ptr(sd)))
+        _check(_lib.ab_mlp_get(self._h, slot, _ptr(p), _ptr(mu), _ptr(sd)))
         return p, mu, sd
 
-    def clear_model(self):
-        _check(_lib.ab_mlp_clear(self._h))
+    def clear_model(self, slot: Optional[int] = None):
+        for s_ in ((0, 1) if slot is None else (slot,)):
+            _check(_lib.ab_mlp_clear(self._h, s_))
 
     def load_model(self, path: str) -> dict:
+        """Loads a model file: {"single": {...}, "batch": {...}} (either may be
+        absent) with params/mean/std each."""
         import json
         d = json.load(open(path))
-        self.set_model(d["params"], d["mean"], d["std"])
+        for slot, name in ((MODEL_SINGLE, "single"), (MODEL_BATCH, "batch")):
+            if name in d:
+                self.set_model(d[name]["params"], d[name]["mean"], d[name]["std"], slot)
         return d
 
-    def save_model(self, path: str, meta: Optional[dict] = None):
-        import json
-        p, mu, sd = self.get_model()
-        json.dump({"dims": [NFEATURES, 64, 64, 2], "params": p.tolist(), "mean": mu.tolist(), "std": sd.tolist(),
-                   "meta": meta or {}}, open(path, "w"))
+    def model_dict(self, slot: int) -> dict:
+        p, mu, sd = self.get_model(slot)
+        return {"dims": [NFEATURES, 64, 64, 2], "params": p.tolist(), "mean": mu.tolist(), "std": sd.tolist()}
 
-    def predict(self, x):
+    def predict(self, x, slot: int = 0):
         xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEATURES)
         y = np.zeros((xa.shape[0], 2), np.float32)
         sel = np.zeros(max(xa.shape[0], 1), np.uint8)
-        _check(_lib.ab_mlp_predict(self._h, _ptr(xa), xa.shape[0], _ptr(y), _ptr(sel)))
+        _check(_lib.ab_mlp_predict(self._h, slot, _ptr(xa), xa.shape[0], _ptr(y), _ptr(sel)))
         return y, sel[: xa.shape[0]]
 
-    def model_grad(self, x, y):
+    def model_grad(self, x, y, slot: int = 0):
         xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEATURES)
         ya = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, 2)
         g = np.zeros(_lib.ab_mlp_nparams(), np.float32)
         loss = C.c_float()
-        _check(_lib.ab_mlp_grad(self._h, _ptr(xa), _ptr(ya), xa.shape[0], _ptr(g), C.byref(loss)))
+        _check(_lib.ab_mlp_grad(self._h, slot, _ptr(xa), _ptr(ya), xa.shape[0], _ptr(g), C.byref(loss)))
         return loss.value, g
 
-    def train_model(self, x, y, epochs: int = 30, batch: int = 256, lr: float = 1e-3, seed: int = 1):
+    def train_model(self, x, y, epochs: int = 30, batch: int = 256, lr: float = 1e-3, seed: int = 1,
+                    slot: int = 0):
         xa = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, NFEATURES)
         ya = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, 2)
         curve = np.zeros(max(epochs, 1), np.float32)
-        _check(_lib.ab_mlp_train(self._h, _ptr(xa), _ptr(ya), xa.shape[0], epochs, batch, lr, seed, _ptr(curve)))
+        _check(_lib.ab_mlp_train(self._h, slot, _ptr(xa), _ptr(ya), xa.shape[0], epochs, batch, lr, seed,
+                                 _ptr(curve)))
         return curve[:epochs]
 
     def features(self, d_keys, qoff) -> Tuple[np.ndarray, np.ndarray]:

@@ -2,13 +2,20 @@
 
 Labels come from the method itself, exactly as the reference harness does it
 (test/adsfilter_q_all.cu:414-438): every query is run through BOTH paths and
-each is timed.  Here each run is one synchronized single-query call
-(enqueue -> stream synchronize, host wall clock), best of `repeats`.  The
-model (12 features -> 64 -> 64 -> 2, log milliseconds of dense and sparse) is
-trained by the library's own HIP kernels (ab_mlp_train); the threshold rule
-(dense iff M(q) >= tau * R, tau fit on the same training timings) is both the
-fallback and the yardstick: regret = sum of the chosen path's time over the
-sum of the per-query best time, minus one.
+each is timed.  Two label sets, one per model slot:
+
+* single (MODEL_SINGLE): one synchronized single-query call per path
+  (enqueue -> stream synchronize, host wall clock), best of `repeats`;
+* batch (MODEL_BATCH): the query's share of a batch, from HIP-event device
+  times -- sparse: its own zero-fill and union kernels run alone; dense: the
+  tile and mask kernels of a random group of up to 64 queries containing it,
+  divided by the group size (one pass over the index serves the group).
+
+Each model (12 features -> 64 -> 64 -> 2, log milliseconds of dense and
+sparse) is trained by the library's own HIP kernels (ab_mlp_train); the
+threshold rule (dense iff M(q) >= tau * R, tau fit on the same training
+timings) is both the fallback and the yardstick: regret = sum of the chosen
+path's time over the sum of the per-query best time, minus one.
 
 Run as a module on a GPU host to produce the committed model:
     python -m adabool_amd.costmodel --out adabool_amd/models/cost_model_c4.json
@@ -89,6 +96,47 @@ def time_both(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, repea
     return feats, M, t
 
 
+def time_batch_shares(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, seed: int, repeats: int = 2,
+                      group: int = 64):
+    """Per query: device ms of its share of a batch on each path (see module doc)."""
+    import torch
+
+    nq = qoff.size - 1
+    sw = ix.stride_words
+    dk = torch.from_numpy(keys.view(np.int32)).cuda() if keys.size else torch.zeros(1, dtype=torch.int32,
+                                                                                     device="cuda")
+    bm = torch.empty(group * sw, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    t = np.full((nq, 2), np.inf)
+    ctx.set_timing(True)
+    try:
+        base = dk.data_ptr()
+        for i in range(nq):
+            a, b = int(qoff[i]), int(qoff[i + 1])
+            for _ in range(repeats):
+                ctx.timing(reset=True)
+                ctx.query_batch_device(base + 4 * a, np.array([0, b - a], np.uint64), AB_PATH_SPARSE, bm)
+                k = ctx.timing(reset=True)
+                t[i, 1] = min(t[i, 1], k["sparse_union"][0] + k["sparse_zero"][0])
+        rng = np.random.default_rng(seed)
+        for _ in range(repeats):
+            perm = rng.permutation(nq)
+            for g0 in range(0, nq, group):
+                members = perm[g0:g0 + group]
+                gk = np.concatenate([keys[int(qoff[m]):int(qoff[m + 1])] for m in members])
+                go = np.concatenate([[0], np.cumsum([int(qoff[m + 1] - qoff[m]) for m in members])]).astype(np.uint64)
+                gd = torch.from_numpy(gk.view(np.int32)).cuda() if gk.size else dk
+                torch.cuda.synchronize()
+                ctx.timing(reset=True)
+                ctx.query_batch_device(gd, go, AB_PATH_DENSE, bm)
+                k = ctx.timing(reset=True)
+                share = (k["dense_tiles"][0] + k["dense_mask"][0]) / members.size
+                t[members, 0] = np.minimum(t[members, 0], share)
+    finally:
+        ctx.set_timing(False)
+    return t
+
+
 def fit_threshold(M: np.ndarray, R: np.ndarray, t: np.ndarray) -> float:
     """tau minimizing total time of the rule dense iff M >= tau * R."""
     frac = M.astype(np.float64) / R.astype(np.float64)
@@ -111,6 +159,14 @@ def fit_threshold(M: np.ndarray, R: np.ndarray, t: np.ndarray) -> float:
 def regret(choice_dense: np.ndarray, t: np.ndarray) -> float:
     chosen = np.where(choice_dense, t[:, 0], t[:, 1])
     return float(chosen.sum() / np.minimum(t[:, 0], t[:, 1]).sum() - 1.0)
+
+
+def _train(ctx: Context, slot: int, F, T, tr, epochs: int):
+    ctx.clear_model(slot)
+    Y = np.log(T).astype(np.float32)
+    curve = ctx.train_model(F[tr], Y[tr], epochs=epochs, batch=256, lr=2e-3, seed=7 + slot, slot=slot)
+    _, sel = ctx.predict(F, slot=slot)
+    return sel == AB_PATH_DENSE, curve
 
 
 def main(argv: List[str] = None):
@@ -136,43 +192,46 @@ def main(argv: List[str] = None):
         del pk, pv
         ctx = Context(ix)
         keys, qoff = query_mix(spec, a.queries, spec.seed + 17)
-        f, M, t = time_both(ctx, ix, keys, qoff, a.repeats)
+        f, M, t1 = time_both(ctx, ix, keys, qoff, a.repeats)
+        tb = time_batch_shares(ctx, ix, keys, qoff, spec.seed, a.repeats)
         R = np.full(M.size, ix.info["nrec_total"], np.uint64)
-        rows.append((f, M, R, t, np.full(M.size, held)))
+        rows.append((f, M, R, t1, tb, np.full(M.size, held)))
         print(f"[costmodel] {spec.name}: R={ix.info['nrec_total']} {a.queries} queries timed in "
-              f"{time.time() - t0:.1f}s; dense wins {(t[:, 0] < t[:, 1]).mean():.2f}", file=sys.stderr, flush=True)
+              f"{time.time() - t0:.1f}s; dense wins single {(t1[:, 0] < t1[:, 1]).mean():.2f} "
+              f"batch {(tb[:, 0] < tb[:, 1]).mean():.2f}", file=sys.stderr, flush=True)
         ctx.close()
         ix.close()
     F = np.concatenate([r[0] for r in rows])
     M = np.concatenate([r[1] for r in rows])
     R = np.concatenate([r[2] for r in rows])
-    T = np.concatenate([r[3] for r in rows])
-    H = np.concatenate([r[4] for r in rows])
+    T1 = np.concatenate([r[3] for r in rows])
+    TB = np.concatenate([r[4] for r in rows])
+    H = np.concatenate([r[5] for r in rows])
     os.makedirs(os.path.dirname(os.path.abspath(a.data_out)), exist_ok=True)
-    np.savez_compressed(a.data_out, features=F, M=M, R=R, t_ms=T, heldout=H)
+    np.savez_compressed(a.data_out, features=F, M=M, R=R, t_single_ms=T1, t_batch_ms=TB, heldout=H)
     tr, te = ~H, H
-    tau = fit_threshold(M[tr], R[tr], T[tr])
-    # train on the GPU with the library's kernels (any index's context works)
     spec = filter_spec(int(TRAIN_INDICES[0][0] * a.scale), TRAIN_INDICES[0][1], 0)
     pk, pv = configs.make_filter(spec)
     ix = Index(pk, pv, spec.nkeys, spec.nvalues)
     ctx = Context(ix)
-    Y = np.log(T).astype(np.float32)
-    curve = ctx.train_model(F[tr], Y[tr], epochs=a.epochs, batch=256, lr=2e-3, seed=7)
-    pred, sel = ctx.predict(F)
-    learned = sel == AB_PATH_DENSE
-    thr = (M.astype(np.float64) >= tau * R.astype(np.float64))
-    res = {
-        "tau": tau,
-        "train_regret_threshold": regret(thr[tr], T[tr]), "train_regret_learned": regret(learned[tr], T[tr]),
-        "heldout_regret_threshold": regret(thr[te], T[te]), "heldout_regret_learned": regret(learned[te], T[te]),
-        "heldout_queries": int(te.sum()), "train_queries": int(tr.sum()),
-        "final_loss": float(curve[-1]), "first_loss": float(curve[0]),
-    }
-    print(json.dumps(res), flush=True)
+    out = {"meta": {"trained_on": [list(x) for x in TRAIN_INDICES], "heldout": list(HELDOUT_INDEX),
+                    "train_queries": int(tr.sum()), "heldout_queries": int(te.sum())}}
+    for slot, name, T in ((0, "single", T1), (1, "batch", TB)):
+        tau = fit_threshold(M[tr], R[tr], T[tr])
+        learned, curve = _train(ctx, slot, F, T, tr, a.epochs)
+        thr = M.astype(np.float64) >= tau * R.astype(np.float64)
+        res = {
+            "tau": tau,
+            "train_regret_threshold": regret(thr[tr], T[tr]), "train_regret_learned": regret(learned[tr], T[tr]),
+            "heldout_regret_threshold": regret(thr[te], T[te]), "heldout_regret_learned": regret(learned[te], T[te]),
+            "first_loss": float(curve[0]), "final_loss": float(curve[-1]),
+            "labels": "log ms; " + ("single-query synchronized wall time" if slot == 0 else
+                                    "device time of the query's share of a batch"),
+        }
+        print(name, json.dumps(res), flush=True)
+        out[name] = dict(ctx.model_dict(slot), meta=res)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
-    ctx.save_model(a.out, meta=dict(res, trained_on=[list(x) for x in TRAIN_INDICES], heldout=list(HELDOUT_INDEX),
-                                    labels="log ms, single-query synchronized wall time, best of repeats"))
+    json.dump(out, open(a.out, "w"))
 
 
 if __name__ == "__main__":

@@ -409,7 +409,7 @@ extern "C" ab_status ab_ctx_create(ab_index *ix, void *stream, ab_ctx **out)
 ab_ctx::~ab_ctx()
 {
     for (DevBuf *b : {&qoff, &sel, &kq, &cnt, &koff, &parts, &feat, &M, &ccnt, &bm, &qlist, &keys, &qbm, &mask,
-                      &mlp_params, &mlp_norm, &mlp_pred})
+                      &mlp[0].params, &mlp[0].norm, &mlp[1].params, &mlp[1].norm})
         b->release();
     for (PinBuf *b : {&hstage, &hkstage, &hsel, &hM})
         b->release();
@@ -654,9 +654,12 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
         {
             TimedSpan ts(c, AB_K_DECIDE);
             launch_features(d, a.d_keys, qoff_d, nq, ix->R, (float *)c->feat.p, (uint64_t *)c->M.p, s);
-            bool learned = (a.path == AB_PATH_LEARNED || a.path == AB_PATH_AUTO) && c->mlp_loaded;
+            // one dense pass serves a whole group inside a batch: batches are
+            // priced by the batch model, single queries by the single model
+            const int slot = nq > 1 ? 1 : 0;
+            const bool learned = (a.path == AB_PATH_LEARNED || a.path == AB_PATH_AUTO) && c->mlp[slot].loaded;
             if (learned) {
-                AB_TRY(mlp_decide(c, (const float *)c->feat.p, nq, (uint8_t *)c->sel.p));
+                AB_TRY(mlp_decide(c, slot, (const float *)c->feat.p, nq, (uint8_t *)c->sel.p));
             } else {
                 const double tau = nq > 1 ? c->tau_batch : c->tau;
                 launch_decide_threshold((const uint64_t *)c->M.p, nq, tau * (double)ix->R, (uint8_t *)c->sel.p, s);
@@ -795,7 +798,7 @@ static int choose_host(ab_ctx *c, const uint32_t *keys, uint32_t n, int path, ui
     float f[AB_NFEATURES];
     host_features(c->ix->seg8.data(), c->ix->nkeys, c->ix->nvalues, c->ix->R, keys, n, f, M);
     if (path == AB_PATH_DENSE || path == AB_PATH_SPARSE) return path;
-    if ((path == AB_PATH_LEARNED || path == AB_PATH_AUTO) && c->mlp_loaded) return mlp_decide_host(c, f);
+    if ((path == AB_PATH_LEARNED || path == AB_PATH_AUTO) && c->mlp[0].loaded) return mlp_decide_host(c, 0, f);
     return (double)*M >= c->tau * (double)c->ix->R ? AB_PATH_DENSE : AB_PATH_SPARSE;
 }
 

@@ -92,10 +92,14 @@ struct ab_ctx {
     std::vector<hipEvent_t> ev_pool;
     double t_ms[AB_K_NCLASSES] = {0};
     uint64_t t_n[AB_K_NCLASSES] = {0};
-    // cost model
-    bool mlp_loaded = false;
-    std::vector<float> mlp_host;  // params followed by mean[12], std[12]
-    ab::DevBuf mlp_params, mlp_norm, mlp_pred;
+    // cost models: slot 0 prices single queries (wall time of one call),
+    // slot 1 prices a query's share of a batch (device time of its kernels)
+    struct Model {
+        bool loaded = false;
+        std::vector<float> host;  // params followed by mean[12], std[12]
+        ab::DevBuf params, norm;
+    };
+    Model mlp[2];
     ~ab_ctx();
 };
 
@@ -110,8 +114,9 @@ struct TimedSpan {
     ~TimedSpan();
 };
 ab_status run_batch(ab_ctx *c, const BatchArgs &a);
-// Device decision for nq queries from device features (MLP forward kernel).
-ab_status mlp_decide(ab_ctx *c, const float *d_feat, uint32_t nq, uint8_t *d_sel);
-// Host decision for one query's features.
-int mlp_decide_host(const ab_ctx *c, const float *f);
+// Device decision for nq queries from device features with model `slot`
+// (MLP forward kernel).
+ab_status mlp_decide(ab_ctx *c, int slot, const float *d_feat, uint32_t nq, uint8_t *d_sel);
+// Host decision for one query's features with model `slot`.
+int mlp_decide_host(const ab_ctx *c, int slot, const float *f);
 }  // namespace ab

@@ -204,15 +204,16 @@ __global__ __launch_bounds__(kStepNT) void k_mlp_step(float *__restrict__ params
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-ab_status mlp_decide(ab_ctx *c, const float *d_feat, uint32_t nq, uint8_t *d_sel)
+ab_status mlp_decide(ab_ctx *c, int slot, const float *d_feat, uint32_t nq, uint8_t *d_sel)
 {
-    if (!c->mlp_loaded) {
+    const ab_ctx::Model &md = c->mlp[slot];
+    if (!md.loaded) {
         set_error("cost model not loaded");
         return AB_INVALID_ARG;
     }
     if (!nq) return AB_OK;
-    hipLaunchKernelGGL(k_mlp_decide, dim3((nq + 255) / 256), dim3(256), 0, c->stream, (const float *)c->mlp_params.p,
-                       (const float *)c->mlp_norm.p, d_feat, nq, d_sel, (float *)nullptr);
+    hipLaunchKernelGGL(k_mlp_decide, dim3((nq + 255) / 256), dim3(256), 0, c->stream, (const float *)md.params.p,
+                       (const float *)md.norm.p, d_feat, nq, d_sel, (float *)nullptr);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_DEVICE_ERROR;
 }
 
@@ -236,10 +237,11 @@ static void host_fwd(const float *p, const float *norm, const float *f, float (&
     }
 }
 
-int mlp_decide_host(const ab_ctx *c, const float *f)
+int mlp_decide_host(const ab_ctx *c, int slot, const float *f)
 {
     float y[kO];
-    host_fwd(c->mlp_host.data(), c->mlp_host.data() + kParams, f, y);
+    const std::vector<float> &h = c->mlp[slot].host;
+    host_fwd(h.data(), h.data() + kParams, f, y);
     return y[0] < y[1] ? AB_PATH_DENSE : AB_PATH_SPARSE;
 }
 
@@ -263,10 +265,12 @@ using namespace ab;
 
 extern "C" uint32_t ab_mlp_nparams(void) { return (uint32_t)kParams; }
 
-extern "C" ab_status ab_mlp_set(ab_ctx *c, const float *params, const float *mean, const float *stdv)
+static bool bad_slot(int slot) { return slot < 0 || slot > 1; }
+
+extern "C" ab_status ab_mlp_set(ab_ctx *c, int slot, const float *params, const float *mean, const float *stdv)
 {
-    if (!c || !params || !mean || !stdv) {
-        set_error("ab_mlp_set: NULL argument");
+    if (!c || !params || !mean || !stdv || bad_slot(slot)) {
+        set_error("ab_mlp_set: NULL argument or bad slot");
         return AB_INVALID_ARG;
     }
     for (int i = 0; i < kF; ++i)
@@ -279,46 +283,49 @@ extern "C" ab_status ab_mlp_set(ab_ctx *c, const float *params, const float *mea
             set_error("ab_mlp_set: non-finite parameter");
             return AB_INVALID_ARG;
         }
-    c->mlp_host.assign(params, params + kParams);
-    c->mlp_host.insert(c->mlp_host.end(), mean, mean + kF);
-    c->mlp_host.insert(c->mlp_host.end(), stdv, stdv + kF);
-    AB_TRY(c->mlp_params.ensure(kParams * 4));
-    AB_TRY(c->mlp_norm.ensure(2 * kF * 4));
-    HIP_TRY(hipMemcpy(c->mlp_params.p, params, kParams * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->mlp_norm.p, c->mlp_host.data() + kParams, 2 * kF * 4, hipMemcpyHostToDevice));
-    c->mlp_loaded = true;
+    ab_ctx::Model &md = c->mlp[slot];
+    md.host.assign(params, params + kParams);
+    md.host.insert(md.host.end(), mean, mean + kF);
+    md.host.insert(md.host.end(), stdv, stdv + kF);
+    AB_TRY(md.params.ensure(kParams * 4));
+    AB_TRY(md.norm.ensure(2 * kF * 4));
+    HIP_TRY(hipMemcpy(md.params.p, params, kParams * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(md.norm.p, md.host.data() + kParams, 2 * kF * 4, hipMemcpyHostToDevice));
+    md.loaded = true;
     return AB_OK;
 }
 
-extern "C" ab_status ab_mlp_get(ab_ctx *c, float *params, float *mean, float *stdv)
+extern "C" ab_status ab_mlp_get(ab_ctx *c, int slot, float *params, float *mean, float *stdv)
 {
-    if (!c || !c->mlp_loaded) {
-        set_error("ab_mlp_get: no model loaded");
+    if (!c || bad_slot(slot) || !c->mlp[slot].loaded) {
+        set_error("ab_mlp_get: no model loaded in that slot");
         return AB_INVALID_ARG;
     }
+    const ab_ctx::Model &md = c->mlp[slot];
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (params) HIP_TRY(hipMemcpy(params, c->mlp_params.p, kParams * 4, hipMemcpyDeviceToHost));
-    if (mean) memcpy(mean, c->mlp_host.data() + kParams, kF * 4);
-    if (stdv) memcpy(stdv, c->mlp_host.data() + kParams + kF, kF * 4);
+    if (params) HIP_TRY(hipMemcpy(params, md.params.p, kParams * 4, hipMemcpyDeviceToHost));
+    if (mean) memcpy(mean, md.host.data() + kParams, kF * 4);
+    if (stdv) memcpy(stdv, md.host.data() + kParams + kF, kF * 4);
     return AB_OK;
 }
 
-extern "C" ab_status ab_mlp_clear(ab_ctx *c)
+extern "C" ab_status ab_mlp_clear(ab_ctx *c, int slot)
 {
-    if (!c) {
-        set_error("ab_mlp_clear: NULL context");
+    if (!c || bad_slot(slot)) {
+        set_error("ab_mlp_clear: NULL context or bad slot");
         return AB_INVALID_ARG;
     }
-    c->mlp_loaded = false;
+    c->mlp[slot].loaded = false;
     return AB_OK;
 }
 
-extern "C" ab_status ab_mlp_predict(ab_ctx *c, const float *h_x, uint32_t n, float *h_y, uint8_t *h_sel)
+extern "C" ab_status ab_mlp_predict(ab_ctx *c, int slot, const float *h_x, uint32_t n, float *h_y, uint8_t *h_sel)
 {
-    if (!c || !c->mlp_loaded || (n && (!h_x || !h_y))) {
+    if (!c || bad_slot(slot) || !c->mlp[slot].loaded || (n && (!h_x || !h_y))) {
         set_error("ab_mlp_predict: bad argument or no model");
         return AB_INVALID_ARG;
     }
+    const ab_ctx::Model &md = c->mlp[slot];
     if (!n) return AB_OK;
     DevBuf x, y, s;
     AB_TRY(x.ensure((size_t)n * kF * 4));
@@ -327,9 +334,8 @@ extern "C" ab_status ab_mlp_predict(ab_ctx *c, const float *h_x, uint32_t n, flo
     ab_status st = AB_OK;
     if (hipMemcpyAsync(x.p, h_x, (size_t)n * kF * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = AB_DEVICE_ERROR;
     if (st == AB_OK) {
-        hipLaunchKernelGGL(k_mlp_decide, dim3((n + 255) / 256), dim3(256), 0, c->stream,
-                           (const float *)c->mlp_params.p, (const float *)c->mlp_norm.p, (const float *)x.p, n,
-                           (uint8_t *)s.p, (float *)y.p);
+        hipLaunchKernelGGL(k_mlp_decide, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float *)md.params.p,
+                           (const float *)md.norm.p, (const float *)x.p, n, (uint8_t *)s.p, (float *)y.p);
         if (hipMemcpyAsync(h_y, y.p, (size_t)n * kO * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             (h_sel && hipMemcpyAsync(h_sel, s.p, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
             hipStreamSynchronize(c->stream) != hipSuccess)
@@ -344,13 +350,14 @@ extern "C" ab_status ab_mlp_predict(ab_ctx *c, const float *h_x, uint32_t n, flo
 
 // Gradient of the loss over all n samples at the loaded parameters (the
 // backward kernel in gradient-only mode), for checks against a CPU reference.
-extern "C" ab_status ab_mlp_grad(ab_ctx *c, const float *h_x, const float *h_y, uint32_t n, float *h_grad,
-                                 float *h_loss)
+extern "C" ab_status ab_mlp_grad(ab_ctx *c, int slot, const float *h_x, const float *h_y, uint32_t n,
+                                 float *h_grad, float *h_loss)
 {
-    if (!c || !c->mlp_loaded || !n || n > (uint32_t)kMaxBatch || !h_x || !h_y || !h_grad) {
+    if (!c || bad_slot(slot) || !c->mlp[slot].loaded || !n || n > (uint32_t)kMaxBatch || !h_x || !h_y || !h_grad) {
         set_error("ab_mlp_grad: bad argument (1..4096 samples, model loaded)");
         return AB_INVALID_ARG;
     }
+    ab_ctx::Model &md = c->mlp[slot];
     DevBuf x, y, idx, scr, g, l;
     std::vector<uint32_t> id(n);
     for (uint32_t i = 0; i < n; ++i) id[i] = i;
@@ -366,8 +373,8 @@ extern "C" ab_status ab_mlp_grad(ab_ctx *c, const float *h_x, const float *h_y, 
         hipMemcpy(idx.p, id.data(), (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
         st = AB_DEVICE_ERROR;
     if (st == AB_OK) {
-        hipLaunchKernelGGL(k_mlp_step, dim3(1), dim3(kStepNT), 0, c->stream, (float *)c->mlp_params.p, (float *)nullptr,
-                           (float *)nullptr, (const float *)c->mlp_norm.p, (const float *)x.p, (const float *)y.p,
+        hipLaunchKernelGGL(k_mlp_step, dim3(1), dim3(kStepNT), 0, c->stream, (float *)md.params.p, (float *)nullptr,
+                           (float *)nullptr, (const float *)md.norm.p, (const float *)x.p, (const float *)y.p,
                            (const uint32_t *)idx.p, n, (float *)scr.p, (float *)g.p, (float *)l.p, 0, 0.f, 0.f, 0.f,
                            0.f, 0);
         if (hipStreamSynchronize(c->stream) != hipSuccess ||
@@ -384,13 +391,14 @@ extern "C" ab_status ab_mlp_grad(ab_ctx *c, const float *h_x, const float *h_y, 
 // loaded) with Adam on shuffled minibatches; mean/std are taken from the data
 // when no model is loaded.  loss_curve (epochs floats) gets the mean
 // minibatch loss of each epoch.
-extern "C" ab_status ab_mlp_train(ab_ctx *c, const float *h_x, const float *h_y, uint32_t n, uint32_t epochs,
-                                  uint32_t batch, float lr, uint64_t seed, float *loss_curve)
+extern "C" ab_status ab_mlp_train(ab_ctx *c, int slot, const float *h_x, const float *h_y, uint32_t n,
+                                  uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *loss_curve)
 {
-    if (!c || !h_x || !h_y || n == 0 || batch == 0 || batch > (uint32_t)kMaxBatch || !(lr > 0)) {
+    if (!c || bad_slot(slot) || !h_x || !h_y || n == 0 || batch == 0 || batch > (uint32_t)kMaxBatch || !(lr > 0)) {
         set_error("ab_mlp_train: bad argument");
         return AB_INVALID_ARG;
     }
+    ab_ctx::Model &md = c->mlp[slot];
     uint64_t st64 = seed ? seed : 0x9E3779B97F4A7C15ull;
     auto rnd = [&]() {
         st64 ^= st64 << 13;
@@ -398,7 +406,7 @@ extern "C" ab_status ab_mlp_train(ab_ctx *c, const float *h_x, const float *h_y,
         st64 ^= st64 << 17;
         return st64;
     };
-    if (!c->mlp_loaded) {
+    if (!md.loaded) {
         std::vector<float> p(kParams, 0.f), mean(kF, 0.f), sd(kF, 1.f);
         for (int f = 0; f < kF; ++f) {
             double s = 0, s2 = 0;
@@ -424,7 +432,7 @@ extern "C" ab_status ab_mlp_train(ab_ctx *c, const float *h_x, const float *h_y,
             for (uint32_t i = 0; i < n; ++i) s += h_y[(size_t)i * kO + o];
             p[ob3 + o] = (float)(s / n);
         }
-        AB_TRY(ab_mlp_set(c, p.data(), mean.data(), sd.data()));
+        AB_TRY(ab_mlp_set(c, slot, p.data(), mean.data(), sd.data()));
     }
     DevBuf x, y, idx, scr, g, l, m, v;
     AB_TRY(x.ensure((size_t)n * kF * 4));
@@ -450,8 +458,8 @@ extern "C" ab_status ab_mlp_train(ab_ctx *c, const float *h_x, const float *h_y,
         for (uint32_t b = 0; b < nb; ++b) {
             const uint32_t b0 = b * batch, bs = std::min(batch, n - b0);
             ++t;
-            hipLaunchKernelGGL(k_mlp_step, dim3(1), dim3(kStepNT), 0, c->stream, (float *)c->mlp_params.p, (float *)m.p,
-                               (float *)v.p, (const float *)c->mlp_norm.p, (const float *)x.p, (const float *)y.p,
+            hipLaunchKernelGGL(k_mlp_step, dim3(1), dim3(kStepNT), 0, c->stream, (float *)md.params.p, (float *)m.p,
+                               (float *)v.p, (const float *)md.norm.p, (const float *)x.p, (const float *)y.p,
                                (const uint32_t *)idx.p + b0, bs, (float *)scr.p, (float *)g.p, (float *)l.p + b, 1, lr,
                                0.9f, 0.999f, 1e-8f, t);
         }
@@ -465,7 +473,7 @@ extern "C" ab_status ab_mlp_train(ab_ctx *c, const float *h_x, const float *h_y,
         }
     }
     // refresh the host copy used by single-query dispatch
-    HIP_TRY(hipMemcpy(c->mlp_host.data(), c->mlp_params.p, kParams * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(md.host.data(), md.params.p, kParams * 4, hipMemcpyDeviceToHost));
     for (DevBuf *bf : {&x, &y, &idx, &scr, &g, &l, &m, &v}) bf->release();
     return AB_OK;
 }

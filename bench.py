@@ -134,9 +134,17 @@ def main():
     path = {"auto": ab.AB_PATH_AUTO, "threshold": ab.AB_PATH_THRESHOLD, "dense": ab.AB_PATH_DENSE,
             "sparse": ab.AB_PATH_SPARSE, "learned": ab.AB_PATH_LEARNED}[args.path]
     dispatch = "threshold"
-    if path in (ab.AB_PATH_AUTO, ab.AB_PATH_LEARNED) and hasattr(ctx, "load_model") and os.path.exists(args.model):
-        ctx.load_model(args.model)
-        dispatch = "learned"
+    model_meta = None
+    if os.path.exists(args.model):
+        d = ctx.load_model(args.model)
+        model_meta = {k: d[k].get("meta") for k in ("single", "batch") if k in d}
+        # thresholds calibrated on the same timings as the model
+        if "single" in d:
+            ctx.set_threshold(d["single"]["meta"]["tau"])
+        if "batch" in d:
+            ctx.set_batch_threshold(d["batch"]["meta"]["tau"])
+        if path in (ab.AB_PATH_AUTO, ab.AB_PATH_LEARNED):
+            dispatch = "learned"
 
     if args.tau is not None:
         ctx.set_threshold(args.tau)
@@ -165,31 +173,40 @@ def main():
     d_cnt = torch.empty(nq, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
 
-    def step():
-        return ctx.query_batch_device(d_keys, qo, path, d_bm, d_ids, nv, d_cnt)
-
-    for _ in range(args.warmup):
-        paths = step()
-    ctx.sync()
-    ctx.timing(reset=True)
-    ctx.set_timing(True)
-    step_ms = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        paths = step()
+    def run(p, steps, warmup):
+        """warmup + timed steps of the batch on dispatch p; per-class kernel times."""
+        for _ in range(warmup):
+            ctx.query_batch_device(d_keys, qo, p, d_bm, d_ids, nv, d_cnt)
         ctx.sync()
-        step_ms.append((time.perf_counter() - s0) * 1e3)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kt = ctx.timing(reset=True)
-    ctx.set_timing(False)
+        ctx.timing(reset=True)
+        ctx.set_timing(True)
+        step_ms = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s0 = time.perf_counter()
+            pth = ctx.query_batch_device(d_keys, qo, p, d_bm, d_ids, nv, d_cnt)
+            ctx.sync()
+            step_ms.append((time.perf_counter() - s0) * 1e3)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kt = ctx.timing(reset=True)
+        ctx.set_timing(False)
+        return wall, step_ms, kt, pth
+
+    wall, step_ms, kt, paths = run(path, args.steps, args.warmup)
     counts = d_cnt.cpu().numpy().view(np.uint32).astype(np.int64)
     qps = nq * args.steps / wall
     ndense = int((paths == ab.AB_PATH_DENSE).sum())
-    log(f"batched: {qps:.0f} q/s, {wall / args.steps * 1e3:.2f} ms/step, dense {ndense}/{nq}")
+    log(f"batched ({dispatch}): {qps:.0f} q/s, {wall / args.steps * 1e3:.2f} ms/step, dense {ndense}/{nq}")
     log("kernels: " + ", ".join(f"{k}={v[0] / args.steps:.3f}ms/{v[1] // max(args.steps, 1)}" for k, v in kt.items()))
+    thr_cmp = None
+    if dispatch == "learned":
+        tw, tms, _, tp = run(ab.AB_PATH_THRESHOLD, max(2, args.steps // 2), 1)
+        thr_cmp = {"qps": round(nq * max(2, args.steps // 2) / tw, 1),
+                   "ms_per_step": round(tw / max(2, args.steps // 2) * 1e3, 3),
+                   "dense_queries": int((tp == ab.AB_PATH_DENSE).sum())}
+        log(f"batched (threshold tau_batch={ctx.batch_threshold:.5f}): {thr_cmp['qps']:.0f} q/s")
 
     # ---- algorithmic bytes (per query: 4|q| + list bytes + bitmap + 4|R| + 4)
     lb = list_bytes_per_query(ix, qk, qo)
@@ -269,6 +286,9 @@ def main():
                        "per_query_threshold_p50": float(np.percentile(lat1, 50)),
                        "per_query_threshold_p99": float(np.percentile(lat1, 99))},
         "per_query_threshold_qps": round(base_qps, 1),
+        "per_query_threshold_tau": ctx.threshold,
+        "batched_threshold": thr_cmp,
+        "cost_model": model_meta,
         "speedup_vs_per_query_threshold": round(qps / base_qps, 2),
         "job_algorithmic_GBps": round(job_gbs, 1),
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},

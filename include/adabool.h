@@ -248,28 +248,34 @@ ab_status ab_query_features_host(const ab_index *ix, const uint32_t *h_keys, uin
  * Predicts log execution time of the dense (output 0) and sparse (output 1)
  * path from AB_NFEATURES standardized features; the dispatcher takes the
  * smaller.  Parameters: W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64],
- * b3[2] (row-major, ReLU after the hidden layers).  The model lives in the
- * context; AB_PATH_AUTO / AB_PATH_LEARNED use it when loaded and fall back to
- * the threshold rule otherwise.  No reference counterpart: the reference runs
- * both paths per query and has no dispatcher. */
+ * b3[2] (row-major, ReLU after the hidden layers).  A context holds two
+ * models: AB_MODEL_SINGLE prices one query run on its own (ab_query),
+ * AB_MODEL_BATCH prices a query's share of a batch (ab_query_batch with two
+ * or more queries, where one dense pass serves a whole group).
+ * AB_PATH_AUTO / AB_PATH_LEARNED use the model of the call's kind when it is
+ * loaded and fall back to the threshold rule otherwise.  No reference
+ * counterpart: the reference runs both paths per query and has no
+ * dispatcher. */
+enum { AB_MODEL_SINGLE = 0, AB_MODEL_BATCH = 1 };
 uint32_t ab_mlp_nparams(void);
-ab_status ab_mlp_set(ab_ctx *ctx, const float *params, const float *mean, const float *stdv);
-ab_status ab_mlp_get(ab_ctx *ctx, float *params, float *mean, float *stdv);
-ab_status ab_mlp_clear(ab_ctx *ctx);
+ab_status ab_mlp_set(ab_ctx *ctx, int slot, const float *params, const float *mean, const float *stdv);
+ab_status ab_mlp_get(ab_ctx *ctx, int slot, float *params, float *mean, float *stdv);
+ab_status ab_mlp_clear(ab_ctx *ctx, int slot);
 /* Device inference for n host feature rows: h_y gets n*2 predictions, h_sel
  * (may be NULL) the chosen paths. */
-ab_status ab_mlp_predict(ab_ctx *ctx, const float *h_x, uint32_t n, float *h_y, uint8_t *h_sel);
+ab_status ab_mlp_predict(ab_ctx *ctx, int slot, const float *h_x, uint32_t n, float *h_y,
+                         uint8_t *h_sel);
 /* Loss gradient over n (1..4096) samples at the loaded parameters, computed
  * by the training kernel in gradient-only mode (loss = mean over samples and
  * outputs of 0.5*(y_hat - y)^2). */
-ab_status ab_mlp_grad(ab_ctx *ctx, const float *h_x, const float *h_y, uint32_t n, float *h_grad,
-                      float *h_loss);
+ab_status ab_mlp_grad(ab_ctx *ctx, int slot, const float *h_x, const float *h_y, uint32_t n,
+                      float *h_grad, float *h_loss);
 /* Adam training on the GPU over n samples (targets n*2 log times), shuffled
  * minibatches of `batch` (<= 4096) rows for `epochs` epochs.  Starts from the
  * loaded model, or from a seeded initialization with mean/std taken from the
  * data.  loss_curve (may be NULL) receives the mean loss of each epoch. */
-ab_status ab_mlp_train(ab_ctx *ctx, const float *h_x, const float *h_y, uint32_t n, uint32_t epochs,
-                       uint32_t batch, float lr, uint64_t seed, float *loss_curve);
+ab_status ab_mlp_train(ab_ctx *ctx, int slot, const float *h_x, const float *h_y, uint32_t n,
+                       uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *loss_curve);
 
 /* ---- synthetic data ------------------------------------------------------- */
 typedef struct ab_dataset ab_dataset;

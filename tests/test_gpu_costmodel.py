@@ -85,7 +85,7 @@ def test_learned_dispatch_end_to_end(small):
     p[o3 + 0] = -1.0                   # y_dense = -h2[0] + b
     p[o3 + 64 * 2] = 0.0
     p[o3 + 64 * 2 + 1] = -8.0          # y_sparse = -8: dense iff log1p(M) > 8
-    ctx.set_model(p, np.zeros(12, np.float32), np.ones(12, np.float32))
+    ctx.set_model(p, np.zeros(12, np.float32), np.ones(12, np.float32), slot=ab.MODEL_BATCH)
     try:
         bms, ids, counts, paths = ctx.query_batch(qk, qo, ab.AB_PATH_LEARNED)
     finally:
