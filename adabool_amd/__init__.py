@@ -103,6 +103,7 @@ _SIGS = {
     "ab_ctx_set_dense_threads": (C.c_int, [_VP, C.c_uint32]),
     "ab_ctx_set_sparse_chunk": (C.c_int, [_VP, C.c_uint32]),
     "ab_ctx_set_dense_columns": (C.c_int, [_VP, C.c_int]),
+    "ab_ctx_set_batch_mode": (C.c_int, [_VP, C.c_int]),
     "ab_ctx_set_timing": (C.c_int, [_VP, C.c_int]),
     "ab_ctx_timing": (C.c_int, [_VP, _VP, _VP, C.c_int]),
     "ab_query": (C.c_int, [_VP, _VP, C.c_uint32, C.c_int, _VP, _VP, _VP, C.POINTER(C.c_int)]),
@@ -142,6 +143,14 @@ def load_library(path: Optional[str] = None):
     p = path or lib_path()
     if not os.path.exists(p):
         raise ImportError(f"adabool_amd: compiled library not found at {p}; run __graft_entry__.build()")
+    # PyTorch ships its own HIP runtime, which it loads under a different file
+    # name than the libamdhip64.so.7 this library links.  Loading torch first
+    # lets this library bind to that same runtime instance; the other order
+    # starts two HIP runtimes in one process and torch then sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(p)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
@@ -417,19 +426,26 @@ class Context:
     def set_sparse_chunk(self, n: int):
         _check(_lib.ab_ctx_set_sparse_chunk(self._h, n))
 
+    BATCH_HOST, BATCH_FUSED, BATCH_GRAPH = 0, 1, 2
+
+    def set_batch_mode(self, mode: int):
+        """0: host-dispatched launches; 1: device dispatch + one persistent
+        union launch; 2 (default): 1 replayed from a cached hipGraph."""
+        _check(_lib.ab_ctx_set_batch_mode(self._h, mode))
+
     def sync(self):
         _check(_lib.ab_ctx_sync(self._h))
 
     KERNEL_CLASSES = ("decide", "sparse_zero", "sparse_plan", "sparse_union", "dense_mask", "dense_tiles",
-                      "popcount", "write_ids")
+                      "popcount", "write_ids", "fused_union")
 
     def set_timing(self, on: bool = True):
         _check(_lib.ab_ctx_set_timing(self._h, 1 if on else 0))
 
     def timing(self, reset: bool = True) -> dict:
         """{class: (total ms, launches)} since the last reset."""
-        ms = np.zeros(8, np.float64)
-        n = np.zeros(8, np.uint64)
+        ms = np.zeros(len(self.KERNEL_CLASSES), np.float64)
+        n = np.zeros(len(self.KERNEL_CLASSES), np.uint64)
         _check(_lib.ab_ctx_timing(self._h, _ptr(ms), _ptr(n), 1 if reset else 0))
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.KERNEL_CLASSES)}
 

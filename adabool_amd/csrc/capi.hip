@@ -401,15 +401,17 @@ extern "C" ab_status ab_ctx_create(ab_index *ix, void *stream, ab_ctx **out)
     HIP_TRY(hipEventCreateWithFlags(&c->kstage_ev, hipEventDisableTiming));
     // clean query bitmap (1 bit per key) and query masks (u64 per key)
     AB_TRY(c->qbm.ensure(((uint64_t)ix->nkeys + 31) / 32 * 4 + 64));
-    HIP_TRY(hipMemset(c->qbm.p, 0, c->qbm.cap));
+    HIP_TRY(hipMemsetAsync(c->qbm.p, 0, c->qbm.cap, c->stream));
     *out = c.release();
     return AB_OK;
 }
 
 ab_ctx::~ab_ctx()
 {
+    release_graph(this);
     for (DevBuf *b : {&qoff, &sel, &kq, &cnt, &koff, &parts, &feat, &M, &ccnt, &bm, &qlist, &keys, &qbm, &mask,
-                      &mlp[0].params, &mlp[0].norm, &mlp[1].params, &mlp[1].norm})
+                      &mlp[0].params, &mlp[0].norm, &mlp[1].params, &mlp[1].norm, &qgroup, &grp_q, &grp_n, &plan,
+                      &counter, &gmask, &ganybm})
         b->release();
     for (PinBuf *b : {&hstage, &hkstage, &hsel, &hM})
         b->release();
@@ -450,6 +452,7 @@ extern "C" ab_status ab_ctx_set_threshold(ab_ctx *c, double tau)
         return AB_INVALID_ARG;
     }
     c->tau = tau;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -462,6 +465,7 @@ extern "C" ab_status ab_ctx_set_batch_threshold(ab_ctx *c, double tau)
         return AB_INVALID_ARG;
     }
     c->tau_batch = tau;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -474,6 +478,7 @@ extern "C" ab_status ab_ctx_set_dense_group(ab_ctx *c, uint32_t w)
         return AB_INVALID_ARG;
     }
     c->group = w;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -484,6 +489,7 @@ extern "C" ab_status ab_ctx_set_dense_threads(ab_ctx *c, uint32_t t)
         return AB_INVALID_ARG;
     }
     c->dense_threads = t;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -494,6 +500,18 @@ extern "C" ab_status ab_ctx_set_dense_columns(ab_ctx *c, int on)
         return AB_INVALID_ARG;
     }
     c->dense_columns = on != 0;
+    c->cfg_gen++;
+    return AB_OK;
+}
+
+extern "C" ab_status ab_ctx_set_batch_mode(ab_ctx *c, int mode)
+{
+    if (!c || mode < 0 || mode > 2) {
+        set_error("ab_ctx_set_batch_mode: 0, 1 or 2");
+        return AB_INVALID_ARG;
+    }
+    c->batch_mode = mode;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -504,6 +522,7 @@ extern "C" ab_status ab_ctx_set_sparse_chunk(ab_ctx *c, uint32_t items)
         return AB_INVALID_ARG;
     }
     c->sparse_chunk = items;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -578,26 +597,26 @@ extern "C" ab_status ab_ctx_timing(ab_ctx *c, double *ms, uint64_t *n, int reset
 // ---------------------------------------------------------------------------
 // orchestration
 // ---------------------------------------------------------------------------
-static int group_mode(uint32_t B) { return B == 1 ? kModeBitmap : (B <= 32 ? kModeMask32 : kModeMask64); }
+int ab::group_mode(uint32_t B) { return B == 1 ? kModeBitmap : (B <= 32 ? kModeMask32 : kModeMask64); }
 
 constexpr uint32_t kLdsBudget = 65536;
 
 // Largest dense group whose LDS (key staging and row tiles) fits the dynamic
 // LDS budget; the column masks are added when they fit too.
-static uint32_t max_group(const ab_ctx *c)
+uint32_t ab::max_group(const ab_ctx *c)
 {
     uint32_t g = c->group;
     while (g > 1 && dense_lds_bytes(group_mode(g), g, c->ix->tile_shift, c->dense_threads, false) > kLdsBudget) --g;
     return g;
 }
 
-static bool col_fits(const ab_ctx *c, int mode, uint32_t B)
+bool ab::col_fits(const ab_ctx *c, int mode, uint32_t B)
 {
     return c->dense_columns && dense_lds_bytes(mode, B, c->ix->tile_shift, c->dense_threads, true) <= kLdsBudget;
 }
 
 // Waits until the pinned staging area is no longer read by an earlier copy.
-static ab_status stage_wait(ab_ctx *c)
+ab_status ab::stage_wait(ab_ctx *c)
 {
     if (c->stage_pending) {
         HIP_TRY(hipEventSynchronize(c->stage_ev));
@@ -962,7 +981,10 @@ extern "C" ab_status ab_query_batch(ab_ctx *c, const uint32_t *d_keys, const uin
     a.ids_stride = ids_stride;
     a.d_counts = d_counts;
     a.h_paths = h_paths;
-    return run_batch(c, a);
+    // the fused path's grids put queries on the y dimension (at most 65535)
+    // and its persistent kernel is built for 256 and 512 threads
+    if (c->batch_mode == 0 || nq == 0 || nq > 65535 || c->dense_threads < 256) return run_batch(c, a);
+    return run_batch_fused(c, a);
 }
 
 extern "C" ab_status ab_compact_ids(ab_ctx *c, const uint32_t *d_bitmaps, uint32_t nq, uint32_t *d_ids,

@@ -82,6 +82,21 @@ struct ab_ctx {
     ab::PinBuf hstage, hkstage, hsel, hM;
     hipEvent_t stage_ev = nullptr, kstage_ev = nullptr;
     bool stage_pending = false, kstage_pending = false;
+    // device-dispatched batches (ab_query_batch):
+    //   0 host-dispatched launches, 1 one fused persistent union launch with
+    //   device-side dispatch and grouping, 2 = 1 replayed from a cached hipGraph
+    int batch_mode = 2;
+    uint64_t cfg_gen = 1;  // bumped by every setting that changes launch parameters
+    ab::DevBuf qgroup, grp_q, grp_n, plan, counter, gmask, ganybm;
+    uint32_t gmask_groups = 0;  // groups the zeroed mask buffers hold
+    int gmask_mode = -1;
+    struct GraphCache {
+        bool valid = false;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<uint64_t> key;
+    } gc;
+    uint32_t fused_grid = 0;  // work-groups of the last fused launch
     // per-kernel-class timing
     bool timing = false;
     struct Span {
@@ -114,6 +129,16 @@ struct TimedSpan {
     ~TimedSpan();
 };
 ab_status run_batch(ab_ctx *c, const BatchArgs &a);
+// Device-dispatched batch: one fused persistent union launch, optionally
+// replayed from a cached hipGraph (fused.hip).
+ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a);
+void release_graph(ab_ctx *c);
+// Largest dense group width that fits the LDS budget and whether its
+// per-value column masks fit as well.
+uint32_t max_group(const ab_ctx *c);
+int group_mode(uint32_t B);
+bool col_fits(const ab_ctx *c, int mode, uint32_t B);
+ab_status stage_wait(ab_ctx *c);
 // Device decision for nq queries from device features with model `slot`
 // (MLP forward kernel).
 ab_status mlp_decide(ab_ctx *c, int slot, const float *d_feat, uint32_t nq, uint8_t *d_sel);

@@ -1,0 +1,258 @@
+// fused.hip -- device-dispatched batches: features, dispatch decisions, dense
+// grouping and query masks are computed on the device, then ONE persistent
+// union launch takes dense (group, tile) items and sparse chunks from a
+// shared work counter; ids are compacted after it.  Nothing returns to the
+// host, so the whole sequence is captured once per batch shape and buffer set
+// into a hipGraph and replayed (mode 2).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+
+namespace ab {
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));            \
+            return e_ == hipErrorOutOfMemory ? AB_OOM : AB_DEVICE_ERROR;              \
+        }                                                                             \
+    } while (0)
+#define AB_TRY(expr)                \
+    do {                            \
+        ab_status s_ = (expr);      \
+        if (s_ != AB_OK) return s_; \
+    } while (0)
+
+__global__ void k_fill_u8(uint8_t *p, uint32_t n, uint8_t v)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void release_graph(ab_ctx *c)
+{
+    if (c->gc.exec) (void)hipGraphExecDestroy(c->gc.exec);
+    if (c->gc.graph) (void)hipGraphDestroy(c->gc.graph);
+    c->gc = ab_ctx::GraphCache();
+}
+
+namespace {
+
+struct Plan {
+    uint32_t nq;
+    uint64_t K;
+    uint32_t G;
+    int mode;
+    bool use_col;
+    uint32_t grid;  // work-groups of the persistent launch
+    uint64_t mask_stride, any_stride;  // elements / words per group
+    uint32_t *bm;
+    uint64_t stride, words;
+};
+
+// Every device launch of one batch; nothing here allocates or synchronizes,
+// so it can be captured into a graph.
+ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
+{
+    ab_index *ix = c->ix;
+    const DevIndex &d = ix->dev;
+    hipStream_t s = c->stream;
+    const uint32_t nq = p.nq;
+    const uint64_t *qoff = (const uint64_t *)c->qoff.p;
+    uint8_t *sel = (uint8_t *)c->sel.p;
+    HIP_TRY(hipMemcpyAsync(c->qoff.p, c->hstage.p, ((size_t)nq + 1) * 8, hipMemcpyHostToDevice, s));
+    {
+        TimedSpan ts(c, AB_K_DECIDE);
+        if (a.path == AB_PATH_DENSE || a.path == AB_PATH_SPARSE) {
+            hipLaunchKernelGGL(k_fill_u8, dim3((nq + 255) / 256), dim3(256), 0, s, sel, nq, (uint8_t)a.path);
+        } else {
+            launch_features(d, a.d_keys, qoff, nq, ix->R, (float *)c->feat.p, (uint64_t *)c->M.p, s);
+            const int slot = nq > 1 ? 1 : 0;
+            if ((a.path == AB_PATH_LEARNED || a.path == AB_PATH_AUTO) && c->mlp[slot].loaded) {
+                AB_TRY(mlp_decide(c, slot, (const float *)c->feat.p, nq, sel));
+            } else {
+                const double tau = nq > 1 ? c->tau_batch : c->tau;
+                launch_decide_threshold((const uint64_t *)c->M.p, nq, tau * (double)ix->R, sel, s);
+            }
+        }
+        launch_assign_groups(sel, nq, p.G, (uint32_t *)c->qgroup.p, (uint32_t *)c->grp_q.p, (uint32_t *)c->grp_n.p,
+                             (uint32_t *)c->plan.p, s);
+    }
+    {
+        TimedSpan ts(c, AB_K_SPARSE_ZERO);
+        launch_zero_sparse(p.bm, p.stride, p.words, sel, nq, s);
+    }
+    {
+        TimedSpan ts(c, AB_K_DENSE_MASK);
+        launch_mask_groups(p.mode, a.d_keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
+                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 0, s);
+    }
+    if (p.K) {
+        TimedSpan ts(c, AB_K_SPARSE_PLAN);
+        launch_sparse_plan(d, a.d_keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
+        launch_scan_u64((const uint64_t *)c->cnt.p, (uint64_t *)c->koff.p, p.K, (uint64_t *)c->parts.p, s);
+    }
+    {
+        TimedSpan ts(c, AB_K_FUSED_UNION);
+        FusedArgs f{};
+        f.keys = a.d_keys;
+        f.kq = (const uint32_t *)c->kq.p;
+        f.koff = (const uint64_t *)c->koff.p;
+        f.K = p.K;
+        f.chunk = c->sparse_chunk;
+        f.plan = (const uint32_t *)c->plan.p;
+        f.grp_q = (const uint32_t *)c->grp_q.p;
+        f.grp_n = (const uint32_t *)c->grp_n.p;
+        f.masks = c->gmask.p;
+        f.mask_stride = p.mask_stride;
+        f.anybms = (const uint32_t *)c->ganybm.p;
+        f.any_stride = p.any_stride;
+        f.out = p.bm;
+        f.out_stride = p.stride;
+        f.out_words = p.words;
+        f.use_col = p.use_col ? 1 : 0;
+        f.counter = (uint32_t *)c->counter.p;
+        HIP_TRY(launch_union_fused(p.mode, c->dense_threads, d, f, p.G, p.grid, s));
+    }
+    {
+        TimedSpan ts(c, AB_K_DENSE_MASK);
+        launch_mask_groups(p.mode, a.d_keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
+                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 1, s);
+    }
+    if (a.d_ids || a.d_counts) {
+        {
+            TimedSpan ts(c, AB_K_POPCOUNT);
+            launch_popcount(p.bm, p.stride, p.words, nq, (uint32_t *)c->ccnt.p, s);
+        }
+        TimedSpan ts(c, AB_K_WRITE_IDS);
+        launch_write_ids(p.bm, p.stride, p.words, nq, (const uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, a.d_counts,
+                         s);
+    }
+    HIP_TRY(hipGetLastError());
+    return AB_OK;
+}
+
+}  // namespace
+
+ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
+{
+    ab_index *ix = c->ix;
+    hipStream_t s = c->stream;
+    Plan p{};
+    p.nq = a.nq;
+    p.K = a.h_qoff[a.nq];
+    p.G = std::max<uint32_t>(1, max_group(c));
+    p.mode = p.G <= 32 ? kModeMask32 : kModeMask64;
+    p.use_col = col_fits(c, p.mode, p.G);
+    HIP_TRY(fused_grid(p.mode, c->dense_threads, ix->dev, p.G, p.use_col, &p.grid));
+    c->fused_grid = p.grid;
+    p.stride = a.out_stride;
+    p.words = a.out_words;
+    const uint32_t nq = a.nq;
+    const uint32_t ngmax = (nq + p.G - 1) / p.G;
+    const uint64_t nkeys = ix->nkeys;
+    p.mask_stride = (nkeys + 63) / 64 * 64;
+    p.any_stride = (nkeys + 31) / 32 + 16;
+    const size_t esize = p.mode == kModeMask32 ? 4 : 8;
+
+    // ---- every buffer exists before anything is enqueued (capture-safe) ----
+    const uint64_t nchunk = (p.words + kCompactWords - 1) / kCompactWords;
+    std::vector<void *> before;
+    for (DevBuf *b : {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan, &c->counter,
+                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm})
+        before.push_back(b->p);
+    AB_TRY(c->qoff.ensure(((size_t)nq + 1) * 8));
+    AB_TRY(c->sel.ensure(nq));
+    AB_TRY(c->feat.ensure((size_t)nq * AB_NFEATURES * 4));
+    AB_TRY(c->M.ensure((size_t)nq * 8));
+    AB_TRY(c->qgroup.ensure((size_t)nq * 4));
+    AB_TRY(c->grp_q.ensure((size_t)ngmax * 64 * 4));
+    AB_TRY(c->grp_n.ensure((size_t)ngmax * 4 + 4));
+    AB_TRY(c->plan.ensure(16));
+    AB_TRY(c->counter.ensure(16));
+    AB_TRY(c->kq.ensure(p.K * 4 + 4));
+    AB_TRY(c->cnt.ensure(p.K * 8 + 8));
+    AB_TRY(c->koff.ensure((p.K + 1) * 8));
+    AB_TRY(c->parts.ensure(scan_parts_needed(p.K) * 8 + 16));
+    AB_TRY(c->ccnt.ensure((size_t)nq * nchunk * 4 + 4));
+    p.bm = a.d_bitmaps;
+    if (!p.bm) {
+        AB_TRY(c->bm.ensure((size_t)nq * p.stride * 4));
+        p.bm = (uint32_t *)c->bm.p;
+    }
+    if (c->gmask_groups < ngmax || c->gmask_mode != p.mode) {
+        // mask buffers start zeroed and are returned to zero by every batch
+        AB_TRY(c->gmask.ensure((size_t)ngmax * p.mask_stride * esize));
+        AB_TRY(c->ganybm.ensure((size_t)ngmax * p.any_stride * 4));
+        // (stream-ordered, never inside a capture: this runs before any)
+        HIP_TRY(hipMemsetAsync(c->gmask.p, 0, c->gmask.cap, s));
+        HIP_TRY(hipMemsetAsync(c->ganybm.p, 0, c->ganybm.cap, s));
+        c->gmask_groups = (uint32_t)std::min<size_t>(c->gmask.cap / (p.mask_stride * esize),
+                                                     c->ganybm.cap / (p.any_stride * 4));
+        c->gmask_mode = p.mode;
+        release_graph(c);
+    }
+    std::vector<void *> after;
+    for (DevBuf *b : {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan, &c->counter,
+                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm})
+        after.push_back(b->p);
+    if (before != after) release_graph(c);
+
+    // ---- stage the offsets (the graph's copy node reads this pinned area) ----
+    AB_TRY(stage_wait(c));
+    AB_TRY(c->hstage.ensure(((size_t)nq + 1) * 8 + 64));
+    memcpy(c->hstage.p, a.h_qoff, ((size_t)nq + 1) * 8);
+
+    const bool graph = c->batch_mode == 2 && !c->timing;
+    if (!graph) {
+        AB_TRY(enqueue(c, a, p));
+    } else {
+        std::vector<uint64_t> key = {(uint64_t)(uintptr_t)a.d_keys, nq, p.K, (uint64_t)a.path, p.grid,
+                                     (uint64_t)(uintptr_t)p.bm, p.stride, p.words, (uint64_t)(uintptr_t)a.d_ids,
+                                     a.ids_stride, (uint64_t)(uintptr_t)a.d_counts, c->cfg_gen,
+                                     (uint64_t)(uintptr_t)c->hstage.p, (uint64_t)(uintptr_t)c->gmask.p,
+                                     (uint64_t)(uintptr_t)c->ganybm.p};
+        for (void *q : after) key.push_back((uint64_t)(uintptr_t)q);
+        if (!(c->gc.valid && c->gc.key == key)) {
+            release_graph(c);
+            HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            ab_status st = enqueue(c, a, p);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (st != AB_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return st;
+            }
+            HIP_TRY(e);
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                HIP_TRY(e);
+            }
+            c->gc.graph = g;
+            c->gc.exec = ex;
+            c->gc.key = key;
+            c->gc.valid = true;
+        }
+        HIP_TRY(hipGraphLaunch(c->gc.exec, s));
+    }
+    HIP_TRY(hipEventRecord(c->stage_ev, s));
+    c->stage_pending = true;
+    if (a.h_paths) {
+        AB_TRY(c->hsel.ensure(nq));
+        HIP_TRY(hipMemcpyAsync(c->hsel.p, c->sel.p, nq, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        memcpy(a.h_paths, c->hsel.p, nq);
+    }
+    return AB_OK;
+}
+
+}  // namespace ab

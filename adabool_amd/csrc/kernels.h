@@ -57,6 +57,39 @@ uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t thr
 hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask,
                               const uint32_t *anybm, const DenseGroup &g, uint32_t *out,
                               uint64_t out_stride, uint64_t out_words, bool use_col, hipStream_t s);
+// Device-dispatched batch: group assignment, sparse zero-fill, group masks.
+void launch_assign_groups(const uint8_t *sel, uint32_t nq, uint32_t G, uint32_t *qgroup, uint32_t *grp_q,
+                          uint32_t *grp_n, uint32_t *plan, hipStream_t s);
+void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq,
+                        hipStream_t s);
+void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, const uint32_t *qgroup,
+                        uint32_t nq, void *masks, uint64_t mask_stride, uint32_t *anybms,
+                        uint64_t any_stride, uint32_t nkeys, int clear, hipStream_t s);
+// Arguments of the persistent union kernel.
+struct FusedArgs {
+    const uint32_t *keys, *kq;   // sparse: keys, query of each key
+    const uint64_t *koff;        // sparse: record prefix over keys (K+1)
+    uint64_t K;
+    uint32_t chunk;              // sparse items per work item
+    const uint32_t *plan;        // [0] dense groups
+    const uint32_t *grp_q;       // 64 query slots per group
+    const uint32_t *grp_n;       // queries per group
+    const void *masks;           // per-group key masks (u32 or u64 per key)
+    uint64_t mask_stride;        // elements per group
+    const uint32_t *anybms;      // per-group 1-bit-per-key filters
+    uint64_t any_stride;         // words per group
+    uint32_t *out;
+    uint64_t out_stride, out_words;
+    int use_col;
+    uint32_t *counter;           // work counter (reset by the launcher)
+};
+uint32_t fused_lds_bytes(int mode, uint32_t G, uint32_t tile_shift, uint32_t threads, bool use_col);
+// Work-groups of the persistent union launch: CUs x resident work-groups per
+// CU (256 or 512 threads).  Queried before any stream capture.
+hipError_t fused_grid(int mode, uint32_t threads, const DevIndex &ix, uint32_t G, bool use_col, uint32_t *grid);
+// Resets the work counter and launches `grid` work-groups.
+hipError_t launch_union_fused(int mode, uint32_t threads, const DevIndex &ix, const FusedArgs &a, uint32_t G,
+                              uint32_t grid, hipStream_t s);
 constexpr uint32_t kCompactWords = 2048;
 void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
                      uint32_t *ccnt, hipStream_t s);

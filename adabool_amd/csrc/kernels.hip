@@ -145,6 +145,7 @@ void launch_fill_zero(void *p, uint64_t bytes, hipStream_t s)
 }
 
 constexpr uint32_t kZeroChunk = 8192;  // words per work-group
+constexpr uint32_t kNoGroup = 0xffffffffu;
 __global__ __launch_bounds__(256) void k_zero_qlist(uint32_t *bm, uint64_t stride, uint64_t words,
                                                     const uint32_t *qlist)
 {
@@ -326,6 +327,59 @@ __device__ __forceinline__ void emit_global(const DevIndex &ix, uint32_t idx, ui
 // split of the total over all lists, instead of the reference's CPU chunking
 // into <=256-record units, src/AdaBool/load_balance_transformer.h:50-69).
 constexpr uint32_t kSparseNT = 256, kSparseWin = 1024;
+
+// Items [c0, c1) of the sparse work: every thread of the work-group takes
+// items c0 + tid, c0 + tid + NT, ...; skoff is an LDS window of kSparseWin+1
+// key offsets, s_lo/s_hi LDS scratch of the search.
+template <int NT>
+__device__ __forceinline__ void sparse_chunk(const DevIndex &ix, const uint32_t *__restrict__ keys,
+                                             const uint32_t *__restrict__ kq, const uint64_t *__restrict__ koff,
+                                             uint64_t K, uint64_t c0, uint64_t c1, uint32_t *__restrict__ out,
+                                             uint64_t out_stride, unsigned long long *skoff,
+                                             unsigned long long *s_lo, unsigned long long *s_hi)
+{
+    uint64_t jw = wg_find<NT>(koff, K, c0, s_lo, s_hi);
+    for (;;) {
+        const uint64_t jn = (jw + kSparseWin < K) ? jw + kSparseWin : K;
+        const uint32_t nwin = (uint32_t)(jn - jw);
+        for (uint32_t i = threadIdx.x; i <= nwin; i += NT) skoff[i] = koff[jw + i];
+        __syncthreads();
+        const uint64_t wlo = skoff[0] > c0 ? skoff[0] : c0;
+        const uint64_t whi = skoff[nwin] < c1 ? skoff[nwin] : c1;
+        for (uint64_t it = wlo + threadIdx.x; it < whi; it += NT) {
+            uint32_t lo = 0, hi = nwin;  // skoff[lo] <= it < skoff[hi]
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (skoff[mid] <= it) lo = mid; else hi = mid;
+            }
+            const uint64_t j = jw + lo;
+            uint32_t r = (uint32_t)(it - skoff[lo]);
+            const uint32_t k = keys[j];
+            const uint32_t q = kq[j];
+            const uint4 *rows = reinterpret_cast<const uint4 *>(ix.seg8 + (uint64_t)k * 8);
+            const uint4 a0 = rows[0], a1 = rows[1], b0 = rows[2], b1 = rows[3];
+            uint32_t *bm = out + (uint64_t)q * out_stride;
+            const uint32_t l0 = b0.x - a0.x, l1 = b0.y - a0.y, l2 = b0.z - a0.z, l3 = b0.w - a0.w,
+                           l4 = b1.x - a1.x;
+            if (r < l0) { emit_global<AB_ARRAY1>(ix, a0.x + r, bm); continue; }
+            r -= l0;
+            if (r < l1) { emit_global<AB_ARRAY4>(ix, a0.y + r, bm); continue; }
+            r -= l1;
+            if (r < l2) { emit_global<AB_ARRAY8>(ix, a0.z + r, bm); continue; }
+            r -= l2;
+            if (r < l3) { emit_global<AB_ARRAY16>(ix, a0.w + r, bm); continue; }
+            r -= l3;
+            if (r < l4) { emit_global<AB_ARRAY32>(ix, a1.x + r, bm); continue; }
+            r -= l4;
+            emit_global<AB_BITSET>(ix, a1.y + r, bm);
+        }
+        const bool done = (skoff[nwin] >= c1) || jn >= K;
+        __syncthreads();
+        if (done) break;
+        jw = jn;
+    }
+}
+
 __global__ __launch_bounds__(kSparseNT) void k_sparse_union(DevIndex ix, const uint32_t *__restrict__ keys,
                                                             const uint32_t *__restrict__ kq,
                                                             const uint64_t *__restrict__ koff, uint64_t K,
@@ -339,46 +393,7 @@ __global__ __launch_bounds__(kSparseNT) void k_sparse_union(DevIndex ix, const u
     const uint64_t nch = (W + CH - 1) / CH;
     for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
         const uint64_t c0 = c * CH, c1 = (c0 + CH < W) ? c0 + CH : W;
-        uint64_t jw = wg_find<kSparseNT>(koff, K, c0, &s_lo, &s_hi);
-        for (;;) {
-            const uint64_t jn = (jw + kSparseWin < K) ? jw + kSparseWin : K;
-            const uint32_t nwin = (uint32_t)(jn - jw);
-            for (uint32_t i = threadIdx.x; i <= nwin; i += kSparseNT) skoff[i] = koff[jw + i];
-            __syncthreads();
-            const uint64_t wlo = skoff[0] > c0 ? skoff[0] : c0;
-            const uint64_t whi = skoff[nwin] < c1 ? skoff[nwin] : c1;
-            for (uint64_t it = wlo + threadIdx.x; it < whi; it += kSparseNT) {
-                uint32_t lo = 0, hi = nwin;  // skoff[lo] <= it < skoff[hi]
-                while (hi - lo > 1) {
-                    uint32_t mid = (lo + hi) >> 1;
-                    if (skoff[mid] <= it) lo = mid; else hi = mid;
-                }
-                const uint64_t j = jw + lo;
-                uint32_t r = (uint32_t)(it - skoff[lo]);
-                const uint32_t k = keys[j];
-                const uint32_t q = kq[j];
-                const uint4 *rows = reinterpret_cast<const uint4 *>(ix.seg8 + (uint64_t)k * 8);
-                const uint4 a0 = rows[0], a1 = rows[1], b0 = rows[2], b1 = rows[3];
-                uint32_t *bm = out + (uint64_t)q * out_stride;
-                const uint32_t l0 = b0.x - a0.x, l1 = b0.y - a0.y, l2 = b0.z - a0.z, l3 = b0.w - a0.w,
-                               l4 = b1.x - a1.x;
-                if (r < l0) { emit_global<AB_ARRAY1>(ix, a0.x + r, bm); continue; }
-                r -= l0;
-                if (r < l1) { emit_global<AB_ARRAY4>(ix, a0.y + r, bm); continue; }
-                r -= l1;
-                if (r < l2) { emit_global<AB_ARRAY8>(ix, a0.z + r, bm); continue; }
-                r -= l2;
-                if (r < l3) { emit_global<AB_ARRAY16>(ix, a0.w + r, bm); continue; }
-                r -= l3;
-                if (r < l4) { emit_global<AB_ARRAY32>(ix, a1.x + r, bm); continue; }
-                r -= l4;
-                emit_global<AB_BITSET>(ix, a1.y + r, bm);
-            }
-            const bool done = (skoff[nwin] >= c1) || jn >= K;
-            __syncthreads();
-            if (done) break;
-            jw = jn;
-        }
+        sparse_chunk<kSparseNT>(ix, keys, kq, koff, K, c0, c1, out, out_stride, skoff, &s_lo, &s_hi);
     }
 }
 
@@ -646,23 +661,22 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
     }
 }
 
-// Dynamic LDS: per-wave key staging (1024 u32 per wave), B query row tiles,
-// and for query groups one column mask per value of the tile.
+// One value tile for a group of B queries (output slots qids[0..B)).
+// lds: per-wave key staging (1024 u32 per wave), B query row tiles, and for
+// query groups with use_col one column mask per value of the tile.
 template <int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__restrict__ mask,
-                                                    const uint32_t *__restrict__ anybm, DenseGroup g,
-                                                    uint32_t *__restrict__ out, uint64_t out_stride,
-                                                    uint64_t out_words, int use_col)
+__device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, const void *__restrict__ mask,
+                                           const uint32_t *__restrict__ anybm, uint32_t B,
+                                           const uint32_t *qids, uint32_t *__restrict__ out, uint64_t out_stride,
+                                           uint64_t out_words, int use_col, uint32_t *lds)
 {
     using M = typename MaskOf<MODE>::T;
     constexpr uint32_t NW = NT / 64;
-    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     uint32_t *tiles = lds + NW * kBlockRecs;
-    const uint32_t tile = blockIdx.x;
     const uint32_t tv = 1u << ix.tile_shift, vmask = tv - 1;
     const uint32_t tw = tv >> 5;
     const uint32_t tile_word0 = tile * tw;
-    const uint32_t ntw = g.B * tw;
+    const uint32_t ntw = B * tw;
     M *col = (MODE != kModeBitmap && use_col) ? reinterpret_cast<M *>(tiles + ntw) : nullptr;
     for (uint32_t i = threadIdx.x; i < ntw; i += NT) tiles[i] = 0;
     if (col)
@@ -721,11 +735,11 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
             const unsigned long long qv = (unsigned long long)col[gi * 64 + lane];
             if (__ballot(qv != 0) == 0) continue;
             unsigned long long mine = 0;
-            for (uint32_t b = 0; b < g.B; ++b) {
+            for (uint32_t b = 0; b < B; ++b) {
                 const unsigned long long bal = __ballot((qv >> b) & 1ull);
                 if (lane == b) mine = bal;
             }
-            if (lane < g.B) {
+            if (lane < B) {
                 tiles[lane * tw + 2 * gi] |= (uint32_t)mine;
                 tiles[lane * tw + 2 * gi + 1] |= (uint32_t)(mine >> 32);
             }
@@ -737,16 +751,26 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
     if ((out_stride & 3) == 0 && (nwq & 3) == 0) {
         const uint32_t n4 = (uint32_t)(nwq >> 2), tw4 = tw >> 2;
         const uint4 *src = reinterpret_cast<const uint4 *>(tiles);
-        for (uint32_t i = threadIdx.x; i < g.B * n4; i += NT) {
+        for (uint32_t i = threadIdx.x; i < B * n4; i += NT) {
             const uint32_t b = i / n4, k = i - b * n4;
-            reinterpret_cast<uint4 *>(out + (uint64_t)g.q[b] * out_stride + tile_word0)[k] = src[b * tw4 + k];
+            reinterpret_cast<uint4 *>(out + (uint64_t)qids[b] * out_stride + tile_word0)[k] = src[b * tw4 + k];
         }
     } else {
-        for (uint32_t b = 0; b < g.B; ++b) {
-            uint32_t *dst = out + (uint64_t)g.q[b] * out_stride + tile_word0;
+        for (uint32_t b = 0; b < B; ++b) {
+            uint32_t *dst = out + (uint64_t)qids[b] * out_stride + tile_word0;
             for (uint32_t i = threadIdx.x; i < nwq; i += NT) dst[i] = tiles[b * tw + i];
         }
     }
+}
+
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__restrict__ mask,
+                                                    const uint32_t *__restrict__ anybm, DenseGroup g,
+                                                    uint32_t *__restrict__ out, uint64_t out_stride,
+                                                    uint64_t out_words, int use_col)
+{
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    dense_tile<MODE, NT>(ix, blockIdx.x, mask, anybm, g.B, g.q, out, out_stride, out_words, use_col, lds);
 }
 
 uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t threads, bool use_col)
@@ -783,6 +807,205 @@ hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, co
     if (mode == kModeMask32)
         return launch_dense_mode<kModeMask32>(threads, ix, mask, anybm, g, out, out_stride, out_words, use_col, s);
     return launch_dense_mode<kModeMask64>(threads, ix, mask, anybm, g, out, out_stride, out_words, use_col, s);
+}
+
+// ---------------------------------------------------------------------------
+// device-dispatched batches: group assignment, per-group query masks, and one
+// persistent union launch over dense (group, tile) items and sparse chunks
+// ---------------------------------------------------------------------------
+// Dense queries in query order get rank r -> group r / G, slot r % G; qgroup[q]
+// = group * 64 + slot (kNoGroup for sparse queries); plan = {groups, dense}.
+__global__ __launch_bounds__(1024) void k_assign_groups(const uint8_t *__restrict__ sel, uint32_t nq, uint32_t G,
+                                                        uint32_t *qgroup, uint32_t *grp_q, uint32_t *grp_n,
+                                                        uint32_t *plan)
+{
+    __shared__ uint32_t sh[16];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    uint32_t running = 0;
+    for (uint32_t base = 0; base < nq; base += 1024) {
+        const uint32_t q = base + threadIdx.x;
+        const uint32_t d = (q < nq && sel[q] == AB_PATH_DENSE) ? 1u : 0u;
+        const uint32_t incl = wave_incl_scan(d);
+        if (lane == 63) sh[wave] = incl;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        for (uint32_t w = 0; w < 16; ++w) {
+            if (w < wave) before += sh[w];
+            all += sh[w];
+        }
+        __syncthreads();
+        const uint32_t rank = running + before + incl - d;
+        if (q < nq) {
+            if (d) {
+                const uint32_t g = rank / G, b = rank - g * G;
+                qgroup[q] = g * 64 + b;
+                grp_q[g * 64 + b] = q;
+            } else {
+                qgroup[q] = kNoGroup;
+            }
+        }
+        running += all;
+    }
+    const uint32_t ng = (running + G - 1) / G;
+    for (uint32_t g = threadIdx.x; g < ng; g += 1024) grp_n[g] = (running - g * G) < G ? running - g * G : G;
+    if (threadIdx.x == 0) {
+        plan[0] = ng;
+        plan[1] = running;
+    }
+}
+
+void launch_assign_groups(const uint8_t *sel, uint32_t nq, uint32_t G, uint32_t *qgroup, uint32_t *grp_q,
+                          uint32_t *grp_n, uint32_t *plan, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_assign_groups, dim3(1), dim3(1024), 0, s, sel, nq, G, qgroup, grp_q, grp_n, plan);
+}
+
+__global__ __launch_bounds__(256) void k_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words,
+                                                     const uint8_t *__restrict__ sel)
+{
+    const uint32_t q = blockIdx.y;
+    if (sel[q] != AB_PATH_SPARSE) return;
+    const uint64_t w0 = (uint64_t)blockIdx.x * kZeroChunk;
+    const uint64_t w1 = w0 + kZeroChunk < words ? w0 + kZeroChunk : words;
+    uint32_t *dst = bm + q * stride;
+    for (uint64_t i = w0 + threadIdx.x; i < w1; i += 256) dst[i] = 0;
+}
+
+void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq, hipStream_t s)
+{
+    if (!nq || !words) return;
+    const uint32_t gx = (uint32_t)((words + kZeroChunk - 1) / kZeroChunk);
+    hipLaunchKernelGGL(k_zero_sparse, dim3(gx, nq), dim3(256), 0, s, bm, stride, words, sel);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict__ keys,
+                                                     const uint64_t *__restrict__ qoff,
+                                                     const uint32_t *__restrict__ qgroup, void *masks,
+                                                     uint64_t mask_stride, uint32_t *anybms, uint64_t any_stride,
+                                                     uint32_t nkeys, int clear)
+{
+    using M = typename MaskOf<MODE>::T;
+    const uint32_t q = blockIdx.y;
+    const uint32_t qg = qgroup[q];
+    if (qg == kNoGroup) return;
+    const uint32_t g = qg >> 6, b = qg & 63;
+    M *mask = reinterpret_cast<M *>(masks) + (uint64_t)g * mask_stride;
+    uint32_t *any = anybms + (uint64_t)g * any_stride;
+    const uint64_t a = qoff[q], e = qoff[q + 1];
+    for (uint64_t j = a + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < e; j += (uint64_t)gridDim.x * 256) {
+        const uint32_t k = keys[j];
+        if (k >= nkeys) continue;
+        if (clear) {
+            any[k >> 5] = 0;
+            mask[k] = 0;
+        } else {
+            atomicOr(&any[k >> 5], 1u << (k & 31));
+            atomicOr(&mask[k], (M)1 << b);
+        }
+    }
+}
+
+void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, const uint32_t *qgroup, uint32_t nq,
+                        void *masks, uint64_t mask_stride, uint32_t *anybms, uint64_t any_stride, uint32_t nkeys,
+                        int clear, hipStream_t s)
+{
+    if (!nq) return;
+    dim3 grid(8, nq);
+    if (mode == kModeMask32)
+        hipLaunchKernelGGL(k_mask_groups<kModeMask32>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks, mask_stride,
+                           anybms, any_stride, nkeys, clear);
+    else
+        hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks, mask_stride,
+                           anybms, any_stride, nkeys, clear);
+}
+
+__global__ void k_set_u32(uint32_t *p, uint32_t v)
+{
+    if (threadIdx.x == 0) *p = v;
+}
+
+// Persistent: every work-group claims items from one counter -- first the
+// dense (group, tile) items of all groups, then the sparse chunks -- until
+// none are left.  Dense items come first: they are the long ones.
+template <int MODE, int NT>
+__global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
+{
+    using M = typename MaskOf<MODE>::T;
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    __shared__ uint32_t s_item;
+    __shared__ unsigned long long s_lo, s_hi;
+    const uint32_t ngroups = a.plan[0];
+    const uint64_t nd = (uint64_t)ngroups * ix.ntiles;
+    const uint64_t W = a.K ? a.koff[a.K] : 0;
+    const uint64_t ns = (W + a.chunk - 1) / a.chunk;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const uint64_t item = s_item;
+        __syncthreads();
+        if (item < nd) {
+            const uint32_t g = (uint32_t)(item / ix.ntiles), tile = (uint32_t)(item - (uint64_t)g * ix.ntiles);
+            dense_tile<MODE, NT>(ix, tile, reinterpret_cast<const M *>(a.masks) + (uint64_t)g * a.mask_stride,
+                                 a.anybms + (uint64_t)g * a.any_stride, a.grp_n[g], a.grp_q + (uint64_t)g * 64,
+                                 a.out, a.out_stride, a.out_words, a.use_col, lds);
+        } else if (item < nd + ns) {
+            const uint64_t c0 = (item - nd) * a.chunk, c1 = (c0 + a.chunk < W) ? c0 + a.chunk : W;
+            sparse_chunk<NT>(ix, a.keys, a.kq, a.koff, a.K, c0, c1, a.out, a.out_stride,
+                             reinterpret_cast<unsigned long long *>(lds), &s_lo, &s_hi);
+        } else {
+            break;
+        }
+        __syncthreads();
+    }
+}
+
+uint32_t fused_lds_bytes(int mode, uint32_t G, uint32_t tile_shift, uint32_t threads, bool use_col)
+{
+    const uint32_t d = dense_lds_bytes(mode, G, tile_shift, threads, use_col);
+    const uint32_t sp = (kSparseWin + 1) * 8;
+    return d > sp ? d : sp;
+}
+
+template <int MODE, int NT>
+static hipError_t occupancy_nt(uint32_t lds, int *per)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, k_union_fused<MODE, NT>, NT, lds);
+}
+
+hipError_t fused_grid(int mode, uint32_t threads, const DevIndex &ix, uint32_t G, bool use_col, uint32_t *grid)
+{
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const uint32_t lds = fused_lds_bytes(mode, G, ix.tile_shift, threads, use_col);
+    if (mode == kModeMask32)
+        e = threads == 256 ? occupancy_nt<kModeMask32, 256>(lds, &per) : occupancy_nt<kModeMask32, 512>(lds, &per);
+    else
+        e = threads == 256 ? occupancy_nt<kModeMask64, 256>(lds, &per) : occupancy_nt<kModeMask64, 512>(lds, &per);
+    if (e != hipSuccess) return e;
+    *grid = (uint32_t)(cus * (per < 1 ? 1 : per));
+    return hipSuccess;
+}
+
+hipError_t launch_union_fused(int mode, uint32_t threads, const DevIndex &ix, const FusedArgs &a, uint32_t G,
+                              uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, a.counter, 0u);
+    const uint32_t lds = fused_lds_bytes(mode, G, ix.tile_shift, threads, a.use_col != 0);
+    if (mode == kModeMask32) {
+        if (threads == 256)
+            hipLaunchKernelGGL((k_union_fused<kModeMask32, 256>), dim3(grid), dim3(256), lds, s, ix, a);
+        else
+            hipLaunchKernelGGL((k_union_fused<kModeMask32, 512>), dim3(grid), dim3(512), lds, s, ix, a);
+    } else {
+        if (threads == 256)
+            hipLaunchKernelGGL((k_union_fused<kModeMask64, 256>), dim3(grid), dim3(256), lds, s, ix, a);
+        else
+            hipLaunchKernelGGL((k_union_fused<kModeMask64, 512>), dim3(grid), dim3(512), lds, s, ix, a);
+    }
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

@@ -292,6 +292,7 @@ extern "C" ab_status ab_mlp_set(ab_ctx *c, int slot, const float *params, const 
     HIP_TRY(hipMemcpy(md.params.p, params, kParams * 4, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(md.norm.p, md.host.data() + kParams, 2 * kF * 4, hipMemcpyHostToDevice));
     md.loaded = true;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -316,6 +317,7 @@ extern "C" ab_status ab_mlp_clear(ab_ctx *c, int slot)
         return AB_INVALID_ARG;
     }
     c->mlp[slot].loaded = false;
+    c->cfg_gen++;
     return AB_OK;
 }
 
@@ -474,6 +476,7 @@ extern "C" ab_status ab_mlp_train(ab_ctx *c, int slot, const float *h_x, const f
     }
     // refresh the host copy used by single-query dispatch
     HIP_TRY(hipMemcpy(md.host.data(), md.params.p, kParams * 4, hipMemcpyDeviceToHost));
+    c->cfg_gen++;
     for (DevBuf *bf : {&x, &y, &idx, &scr, &g, &l, &m, &v}) bf->release();
     return AB_OK;
 }

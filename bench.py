@@ -63,6 +63,24 @@ def _segs(ix, t):
     return _SEG_CACHE[key]
 
 
+# kernel class -> kernel symbol prefix in rocprofv3 output
+KERNEL_SYMBOL = {"fused_union": "ab::k_union_fused", "write_ids": "ab::k_write_ids", "dense_tiles": "ab::k_dense_tiles",
+                 "sparse_union": "ab::k_sparse_union", "popcount": "ab::k_popcount", "decide": "ab::k_features"}
+
+
+def measured_traffic(path: str, kclass: str):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, scaled against a plain
+    copy) of the kernel behind `kclass`, from a committed rocprofv3 summary."""
+    if not path or not os.path.exists(path) or kclass not in KERNEL_SYMBOL:
+        return None, None
+    d = json.load(open(path))
+    cands = [(v["dispatches"], k, v) for k, v in d.get("kernels", {}).items() if k.startswith(KERNEL_SYMBOL[kclass])]
+    if not cands:
+        return None, None
+    _, name, v = max(cands)
+    return float(v["fetch_bytes_median"] + v["write_bytes_median"]), f"{os.path.relpath(path, ROOT)}: {name} median"
+
+
 def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float):
     """Oracle (plain C, CSR-by-key union -> bitmap + ids) on the host cores,
     over a bounded prefix of the same queries; also checks the GPU counts of
@@ -103,6 +121,10 @@ def main():
     ap.add_argument("--group", type=int, default=None, help="dense group width")
     ap.add_argument("--no-columns", action="store_true", help="row-only dense groups")
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB copied once by a plain copy kernel (PMC scale)")
+    ap.add_argument("--batch-mode", type=int, default=2, choices=[0, 1, 2],
+                    help="0 host dispatch, 1 device dispatch + fused launch, 2 = 1 replayed from a hipGraph")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c2_batch_step.json"),
+                    help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (roofline traffic)")
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     args = ap.parse_args()
 
@@ -156,6 +178,7 @@ def main():
         ctx.set_dense_group(args.group)
     if args.no_columns:
         ctx.set_dense_columns(False)
+    ctx.set_batch_mode(args.batch_mode)
     if args.calib_copy:
         n = args.calib_copy * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.int32, device="cuda")
@@ -174,12 +197,10 @@ def main():
     torch.cuda.synchronize()
 
     def run(p, steps, warmup):
-        """warmup + timed steps of the batch on dispatch p; per-class kernel times."""
+        """warmup + timed steps of the batch on dispatch p (no per-kernel events)."""
         for _ in range(warmup):
             ctx.query_batch_device(d_keys, qo, p, d_bm, d_ids, nv, d_cnt)
         ctx.sync()
-        ctx.timing(reset=True)
-        ctx.set_timing(True)
         step_ms = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -189,24 +210,45 @@ def main():
             ctx.sync()
             step_ms.append((time.perf_counter() - s0) * 1e3)
         torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
+        return time.perf_counter() - t0, step_ms, pth
+
+    def kernel_times(p, steps):
+        """Per-kernel-class device time with HIP events on the context stream
+        (a graph-mode context runs these steps as individual launches)."""
+        ctx.timing(reset=True)
+        ctx.set_timing(True)
+        for _ in range(steps):
+            ctx.query_batch_device(d_keys, qo, p, d_bm, d_ids, nv, d_cnt)
+        ctx.sync()
         kt = ctx.timing(reset=True)
         ctx.set_timing(False)
-        return wall, step_ms, kt, pth
+        return kt
 
-    wall, step_ms, kt, paths = run(path, args.steps, args.warmup)
+    wall, step_ms, paths = run(path, args.steps, args.warmup)
     counts = d_cnt.cpu().numpy().view(np.uint32).astype(np.int64)
     qps = nq * args.steps / wall
     ndense = int((paths == ab.AB_PATH_DENSE).sum())
-    log(f"batched ({dispatch}): {qps:.0f} q/s, {wall / args.steps * 1e3:.2f} ms/step, dense {ndense}/{nq}")
-    log("kernels: " + ", ".join(f"{k}={v[0] / args.steps:.3f}ms/{v[1] // max(args.steps, 1)}" for k, v in kt.items()))
+    log(f"batched ({dispatch}, mode {args.batch_mode}): {qps:.0f} q/s, {wall / args.steps * 1e3:.2f} ms/step, "
+        f"dense {ndense}/{nq}")
+    ksteps = max(2, min(args.steps, 5))
+    kt = kernel_times(path, ksteps)
+    log("kernels: " + ", ".join(f"{k}={v[0] / ksteps:.3f}ms/{v[1] // ksteps}" for k, v in kt.items() if v[1]))
     thr_cmp = None
+    nsub = max(2, args.steps // 2)
     if dispatch == "learned":
-        tw, tms, _, tp = run(ab.AB_PATH_THRESHOLD, max(2, args.steps // 2), 1)
-        thr_cmp = {"qps": round(nq * max(2, args.steps // 2) / tw, 1),
-                   "ms_per_step": round(tw / max(2, args.steps // 2) * 1e3, 3),
+        tw, tms, tp = run(ab.AB_PATH_THRESHOLD, nsub, 1)
+        thr_cmp = {"qps": round(nq * nsub / tw, 1), "ms_per_step": round(tw / nsub * 1e3, 3),
                    "dense_queries": int((tp == ab.AB_PATH_DENSE).sum())}
         log(f"batched (threshold tau_batch={ctx.batch_threshold:.5f}): {thr_cmp['qps']:.0f} q/s")
+    mode_cmp = {}
+    for m in (0, 1, 2):
+        if m == args.batch_mode:
+            continue
+        ctx.set_batch_mode(m)
+        mw, _, _ = run(path, nsub, 1)
+        mode_cmp[str(m)] = round(nq * nsub / mw, 1)
+        log(f"batched ({dispatch}, mode {m}): {mode_cmp[str(m)]:.0f} q/s")
+    ctx.set_batch_mode(args.batch_mode)
 
     # ---- algorithmic bytes (per query: 4|q| + list bytes + bitmap + 4|R| + 4)
     lb = list_bytes_per_query(ix, qk, qo)
@@ -223,15 +265,17 @@ def main():
         "popcount": float(stride_b * nq),
         "sparse_zero": float(stride_b * (~dense_mask).sum()),
     }
+    kbytes["fused_union"] = kbytes["dense_tiles"] + kbytes["sparse_union"]
     dom = max(kt, key=lambda k: kt[k][0])
     ms_total, launches = kt[dom]
     roof = None
     if dom in kbytes and launches:
-        per_launch_bytes = kbytes[dom] * args.steps / launches
+        per_launch_bytes = kbytes[dom] * ksteps / launches
         avg_ms = ms_total / launches
         ach = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+        traffic, tsrc = measured_traffic(args.traffic, dom)
         roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": tsrc,
                 "bytes_per_launch": per_launch_bytes, "avg_launch_ms": avg_ms, "launches": launches}
 
     # ---- plain per-query threshold dispatch (one query at a time, synchronized)
@@ -288,10 +332,11 @@ def main():
         "per_query_threshold_qps": round(base_qps, 1),
         "per_query_threshold_tau": ctx.threshold,
         "batched_threshold": thr_cmp,
+        "batch_modes_qps": mode_cmp,
         "cost_model": model_meta,
         "speedup_vs_per_query_threshold": round(qps / base_qps, 2),
         "job_algorithmic_GBps": round(job_gbs, 1),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
+        "kernel_ms_per_step": {k: round(v[0] / ksteps, 4) for k, v in kt.items() if v[1]},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -154,6 +154,15 @@ ab_status ab_ctx_set_dense_columns(ab_ctx *ctx, int on);
 /* Records per work item of the sparse kernel (power of two, 256..8192). */
 ab_status ab_ctx_set_sparse_chunk(ab_ctx *ctx, uint32_t items);
 
+/* How ab_query_batch runs a batch: 0 host dispatch (features and decisions
+ * copied back, one launch per dense group), 1 device dispatch (decisions,
+ * grouping and masks on the device, then ONE persistent launch that takes
+ * dense (group, tile) items and sparse chunks from a shared work counter; no
+ * host round trip), 2 (default) = 1 captured once per batch shape and buffer
+ * set into a hipGraph and replayed.  Per-kernel timing (ab_ctx_set_timing)
+ * runs mode 2 batches as mode 1. */
+ab_status ab_ctx_set_batch_mode(ab_ctx *ctx, int mode);
+
 /* Kernel classes reported by ab_ctx_timing. */
 enum {
     AB_K_DECIDE = 0,      /* device features + dispatch decision */
@@ -164,7 +173,8 @@ enum {
     AB_K_DENSE_TILES = 5,
     AB_K_POPCOUNT = 6,
     AB_K_WRITE_IDS = 7,
-    AB_K_NCLASSES = 8
+    AB_K_FUSED_UNION = 8, /* persistent dense + sparse union (device-dispatched batches) */
+    AB_K_NCLASSES = 9
 };
 /* Brackets every kernel class of later calls with HIP events on the context
  * stream (off by default; on costs a few microseconds per call). */

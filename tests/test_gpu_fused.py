@@ -1,0 +1,168 @@
+"""Batch modes of ab_query_batch against the CPU oracle: host dispatch (0),
+device dispatch with one persistent union launch (1), and the same replayed
+from a cached hipGraph (2).  Graph replays are checked one by one, with other
+device work and synchronizations between them, and after every setting that
+changes launch parameters."""
+import numpy as np
+import pytest
+
+import adabool_amd as ab
+import oracle
+from adabool_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+MODES = (ab.Context.BATCH_HOST, ab.Context.BATCH_FUSED, ab.Context.BATCH_GRAPH)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    import torch
+
+    assert torch.cuda.is_available()
+    k, v = configs.make_filter(configs.C1)
+    ix = ab.Index(k, v, configs.C1.nkeys, configs.C1.nvalues)
+    csr = oracle.CSR(k, v, configs.C1.nkeys, configs.C1.nvalues)
+    return ix, csr
+
+
+def _mixed(seed, nq, nkeys):
+    return ab.gen_queries(seed, nkeys, nq, size_dist=1, p1=60.0, p2=1.3, lo=0, hi=3000, key_dist=1, zipf_s=1.0,
+                          perm_seed=configs.C1.seed)
+
+
+def _verify(csr, qk, qo, res, what):
+    bms, ids, counts, paths = res
+    for i in range(qo.size - 1):
+        want_bm, want_ids = csr.union(qk[qo[i]:qo[i + 1]])
+        assert np.array_equal(bms[i], want_bm), f"{what} q{i} path {paths[i]}: bitmap differs"
+        assert counts[i] == want_ids.size, f"{what} q{i}: count"
+        assert np.array_equal(ids[i], want_ids), f"{what} q{i}: ids differ"
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("path", [ab.AB_PATH_THRESHOLD, ab.AB_PATH_DENSE, ab.AB_PATH_SPARSE])
+def test_modes_match_oracle(c1, mode, path):
+    ix, csr = c1
+    ctx = ab.Context(ix)
+    ctx.set_batch_mode(mode)
+    ctx.set_batch_threshold(0.01)
+    qk, qo = _mixed(0xADAB0201, 48, configs.C1.nkeys)
+    print(f"mode={mode} path={path}", flush=True)
+    res = ctx.query_batch(qk, qo, path)
+    if path == ab.AB_PATH_THRESHOLD:
+        assert (res[3] == ab.AB_PATH_DENSE).any() and (res[3] == ab.AB_PATH_SPARSE).any(), res[3]
+    else:
+        assert (res[3] == path).all()
+    _verify(csr, qk, qo, res, f"mode {mode} path {path}")
+    ctx.close()
+
+
+def test_graph_replays_with_eager_work_between(c1):
+    """Same batch shape (query count and key total) with new keys each time:
+    the cached graph is replayed and must read the new keys and offsets."""
+    import torch
+
+    ix, csr = c1
+    ctx = ab.Context(ix)
+    ctx.set_batch_mode(ab.Context.BATCH_GRAPH)
+    ctx.set_batch_threshold(0.01)
+    qk0, qo0 = _mixed(0xADAB0202, 32, configs.C1.nkeys)
+    rng = np.random.default_rng(5)
+    x = torch.arange(1 << 20, device="cuda", dtype=torch.float32)
+    for rep in range(5):
+        # same offsets, permuted query order and fresh key values of the same total
+        qk = qk0.copy()
+        if rep:
+            qk = rng.integers(0, configs.C1.nkeys, size=qk0.size, dtype=np.uint64).astype(np.uint32)
+            if rep % 2:
+                qk[: qk.size // 2] = qk0[: qk.size // 2]
+        print(f"replay {rep}", flush=True)
+        res = ctx.query_batch(qk, qo0, ab.AB_PATH_THRESHOLD)
+        _verify(csr, qk, qo0, res, f"replay {rep}")
+        # unrelated device work and host synchronization between replays
+        y = (x * (rep + 1)).sum()
+        torch.cuda.synchronize()
+        assert float(y) > 0
+    ctx.close()
+
+
+def test_graph_recapture_on_changes(c1):
+    """Shape, setting and model changes each give a new graph; results stay exact."""
+    ix, csr = c1
+    ctx = ab.Context(ix)
+    ctx.set_batch_mode(ab.Context.BATCH_GRAPH)
+    qk, qo = _mixed(0xADAB0203, 40, configs.C1.nkeys)
+    steps = [
+        ("base", lambda: None),
+        ("tau", lambda: ctx.set_batch_threshold(0.002)),
+        ("group 7", lambda: ctx.set_dense_group(7)),
+        ("group 33", lambda: ctx.set_dense_group(33)),
+        ("threads 256", lambda: ctx.set_dense_threads(256)),
+        ("no columns", lambda: ctx.set_dense_columns(False)),
+        ("chunk 256", lambda: ctx.set_sparse_chunk(256)),
+        ("threads 128 (host dispatch)", lambda: ctx.set_dense_threads(128)),
+    ]
+    for name, act in steps:
+        act()
+        for n in (40, 17, 40):  # a smaller batch in between changes the shape
+            sub_qo = qo[: n + 1]
+            sub_qk = qk[: int(sub_qo[-1])]
+            print(f"recapture {name} n={n}", flush=True)
+            res = ctx.query_batch(sub_qk, sub_qo, ab.AB_PATH_THRESHOLD)
+            _verify(csr, sub_qk, sub_qo, res, f"{name} n={n}")
+    ctx.close()
+
+
+def test_device_buffers_and_timing(c1):
+    """query_batch_device into caller buffers with per-kernel timing on (mode 2
+    then runs as mode 1 and reports the fused union class)."""
+    import torch
+
+    ix, csr = c1
+    ctx = ab.Context(ix)
+    qk, qo = _mixed(0xADAB0204, 24, configs.C1.nkeys)
+    nq = qo.size - 1
+    sw = ix.stride_words
+    dk = torch.from_numpy(qk.view(np.int32)).cuda()
+    bm = torch.empty(nq * sw, dtype=torch.int32, device="cuda")
+    for timing in (False, True):
+        ctx.set_timing(timing)
+        bm.fill_(-1)
+        torch.cuda.synchronize()
+        ctx.query_batch_device(dk, qo, ab.AB_PATH_THRESHOLD, bm)
+        ctx.sync()
+        got = bm.cpu().numpy().view(np.uint32).reshape(nq, sw)
+        for i in range(nq):
+            want_bm, _ = csr.union(qk[qo[i]:qo[i + 1]])
+            assert np.array_equal(got[i], want_bm), f"timing={timing} q{i}"
+        if timing:
+            t = ctx.timing(reset=True)
+            assert t["fused_union"][1] >= 1 and t["fused_union"][0] > 0, t
+            assert t["dense_tiles"][1] == 0 and t["sparse_union"][1] == 0, t
+    ctx.set_timing(False)
+    ctx.close()
+
+
+@pytest.mark.parametrize("ts", [8, 13])
+def test_fused_edge_filter(ts):
+    from .test_gpu_parity import _edge_filter
+
+    k, v, nvalues = _edge_filter()
+    ix = ab.Index(k, v, 12, nvalues, ts)
+    csr = oracle.CSR(k, v, 12, nvalues)
+    rng = np.random.default_rng(ts + 1)
+    qs = [rng.choice(12, size=rng.integers(0, 8), replace=False).astype(np.uint32) for _ in range(70)]
+    qs[0] = np.arange(12, dtype=np.uint32)
+    qk = np.concatenate(qs)
+    qo = np.concatenate([[0], np.cumsum([q.size for q in qs])]).astype(np.uint64)
+    for mode in (ab.Context.BATCH_FUSED, ab.Context.BATCH_GRAPH):
+        ctx = ab.Context(ix)
+        ctx.set_batch_mode(mode)
+        for group in (1, 2, 32, 64):
+            ctx.set_dense_group(group)
+            for path in (ab.AB_PATH_DENSE, ab.AB_PATH_SPARSE):
+                print(f"edge ts={ts} mode={mode} group={group} path={path}", flush=True)
+                res = ctx.query_batch(qk, qo, path)
+                _verify(csr, qk, qo, res, f"ts={ts} mode={mode} group={group} path={path}")
+        ctx.close()
