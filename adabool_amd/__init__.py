@@ -113,6 +113,17 @@ _SIGS = {
     "ab_list_union_sparse_batch": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_uint32, _VP, C.c_uint32]),
     "ab_query_batch": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, _VP, _VP, C.c_uint64, _VP, _VP]),
     "ab_compact_ids": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint64, _VP]),
+    "ab_query_batch_packed": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, _VP, _VP, _VP, _VP, _VP]),
+    "ab_ctx_autotune": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, C.c_uint32, _VP, C.c_uint32, _VP]),
+    "ab_server_config_default": (C.c_int, [_VP]),
+    "ab_server_create": (C.c_int, [_VP, _VP, _VP]),
+    "ab_server_ctx": (_VP, [_VP, C.c_uint32]),
+    "ab_server_submit": (C.c_int, [_VP, _VP, C.c_uint32, _VP]),
+    "ab_server_poll": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_int]),
+    "ab_server_release": (C.c_int, [_VP, C.c_uint64]),
+    "ab_server_stats": (C.c_int, [_VP, _VP, C.c_int]),
+    "ab_server_free": (C.c_int, [_VP]),
+    "ab_server_replay": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_double, C.c_uint64, _VP, _VP, _VP]),
     "ab_query_features": (C.c_int, [_VP, _VP, _VP, C.c_uint32, _VP, _VP]),
     "ab_query_features_host": (C.c_int, [_VP, _VP, C.c_uint32, _VP, _VP]),
     "ab_gen_pairs": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, C.c_double,
@@ -382,15 +393,20 @@ def _torch():
 class Context:
     """Per-stream execution context over an Index (ab_ctx)."""
 
-    def __init__(self, index: Index, stream: Optional[int] = None):
+    def __init__(self, index: Index, stream: Optional[int] = None, _handle=None):
         self.index = index
+        self._owned = _handle is None
+        if _handle is not None:  # a context owned by a Server
+            self._h = _VP(_handle)
+            return
         h = _VP()
         _check(_lib.ab_ctx_create(index._h, stream, C.byref(h)))
         self._h = h
 
     def close(self):
         if getattr(self, "_h", None):
-            _lib.ab_ctx_free(self._h)
+            if self._owned:
+                _lib.ab_ctx_free(self._h)
             self._h = None
 
     def __del__(self):
@@ -483,6 +499,33 @@ class Context:
         paths = np.zeros(max(nq, 1), np.uint8)
         _check(_lib.ab_query_batch(self._h, _ptr(d_keys), _ptr(off), nq, path, _ptr(d_bitmaps), _ptr(d_ids),
                                    ids_stride, _ptr(d_counts), _ptr(paths)))
+        return paths[:nq]
+
+    def autotune(self, d_keys, qoff, path: int = AB_PATH_AUTO, repeats: int = 3) -> list:
+        """Times the launch-parameter grid on this batch and keeps the fastest
+        setting; returns every candidate, fastest first."""
+        off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
+        cap = 64
+        buf = (_TuneConfig * cap)()
+        n = C.c_uint32()
+        _check(_lib.ab_ctx_autotune(self._h, _ptr(d_keys), _ptr(off), off.size - 1, path, repeats, buf, cap,
+                                    C.byref(n)))
+        return [{f: getattr(b, f) for f, _ in _TuneConfig._fields_} for b in buf[: min(cap, n.value)]]
+
+    def apply_tuning(self, cfg: dict):
+        self.set_dense_group(int(cfg["group"]))
+        self.set_dense_threads(int(cfg["dense_threads"]))
+        self.set_sparse_chunk(int(cfg["sparse_chunk"]))
+        self.set_dense_columns(bool(cfg["dense_columns"]))
+
+    def query_batch_packed(self, d_keys, qoff, path: int, d_ids, id_off, d_bitmaps=None, d_counts=None) -> np.ndarray:
+        """Batch with query i's ids written at d_ids + id_off[i] (u32 units)."""
+        off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
+        ido = np.ascontiguousarray(np.asarray(id_off, dtype=np.uint64))
+        nq = off.size - 1
+        paths = np.zeros(max(nq, 1), np.uint8)
+        _check(_lib.ab_query_batch_packed(self._h, _ptr(d_keys), _ptr(off), nq, path, _ptr(d_bitmaps), _ptr(d_ids),
+                                          _ptr(ido), _ptr(d_counts), _ptr(paths)))
         return paths[:nq]
 
     def query_batch(self, keys, qoff, path: int = AB_PATH_THRESHOLD, want_ids: bool = True):
@@ -582,6 +625,114 @@ class Context:
 # ---------------------------------------------------------------------------
 # the reference's own interface (ivf_list and its workers)
 # ---------------------------------------------------------------------------
+class _TuneConfig(C.Structure):
+    _fields_ = [("group", C.c_uint32), ("dense_threads", C.c_uint32), ("sparse_chunk", C.c_uint32),
+                ("dense_columns", C.c_int), ("ms", C.c_double)]
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("max_batch", C.c_uint32), ("max_wait_us", C.c_uint32), ("inflight", C.c_uint32),
+                ("arena_bytes", C.c_uint64), ("path", C.c_int), ("deliver", C.c_int), ("batch_mode", C.c_int)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int), ("path", C.c_int), ("count", C.c_uint32),
+                ("ids", C.c_void_p), ("latency_ms", C.c_double)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("batches", C.c_uint64), ("dense_queries", C.c_uint64),
+                ("mean_batch", C.c_double), ("qps", C.c_double), ("lat_mean_ms", C.c_double),
+                ("lat_p50_ms", C.c_double), ("lat_p90_ms", C.c_double), ("lat_p99_ms", C.c_double),
+                ("lat_max_ms", C.c_double), ("arena_used_bytes", C.c_uint64)]
+
+
+class Server:
+    """Serving loop over a resident index (ab_server): thread-safe submit,
+    memory-based admission, continuous batching on `inflight` streams.
+
+    Settings of the contexts (thresholds, cost models, group width...) are
+    made through `contexts` before the first submission."""
+
+    DELIVER_DEVICE, DELIVER_HOST = 0, 1
+
+    def __init__(self, index: Index, **cfg):
+        self.index = index
+        c = _ServerConfig()
+        _check(_lib.ab_server_config_default(C.byref(c)))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise TypeError(f"unknown server setting {k}")
+            setattr(c, k, v)
+        self.config = {f: getattr(c, f) for f, _ in _ServerConfig._fields_}
+        h = _VP()
+        _check(_lib.ab_server_create(index._h, C.byref(c), C.byref(h)))
+        self._h = h
+        self.contexts = [Context(index, _handle=_lib.ab_server_ctx(h, i)) for i in range(c.inflight)]
+
+    def load_model(self, path: str):
+        for ctx in self.contexts:
+            d = ctx.load_model(path)
+        return d
+
+    def submit(self, keys) -> int:
+        k = _u32(keys)
+        t = C.c_uint64()
+        _check(_lib.ab_server_submit(self._h, _ptr(k), k.size, C.byref(t)))
+        return t.value
+
+    def poll(self, cap: int = 256, wait_ms: int = 0) -> list:
+        """Completed requests: dicts with ticket, status, path, count,
+        ids (numpy copy when results are delivered to the host, else the
+        device address) and latency_ms."""
+        buf = (_Result * cap)()
+        n = C.c_uint32()
+        _check(_lib.ab_server_poll(self._h, buf, cap, C.byref(n), wait_ms))
+        out = []
+        for r in buf[: n.value]:
+            ids = r.ids
+            if self.config["deliver"] == 1 and r.ids and r.status == 0:
+                ids = np.ctypeslib.as_array((C.c_uint32 * r.count).from_address(r.ids)).copy() if r.count else \
+                    np.zeros(0, np.uint32)
+            out.append({"ticket": r.ticket, "status": r.status, "path": r.path, "count": r.count, "ids": ids,
+                        "latency_ms": r.latency_ms})
+        return out
+
+    def release(self, ticket: int):
+        _check(_lib.ab_server_release(self._h, ticket))
+
+    def stats(self, reset: bool = False) -> dict:
+        st = _ServerStats()
+        _check(_lib.ab_server_stats(self._h, C.byref(st), 1 if reset else 0))
+        return {f: getattr(st, f) for f, _ in _ServerStats._fields_}
+
+    def replay(self, keys, qoff, rate_qps: float = 0.0, seed: int = 1):
+        """Open-loop Poisson replay; returns (counts, latency ms from scheduled
+        arrival, wall seconds)."""
+        k = _u32(keys)
+        off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
+        nq = off.size - 1
+        counts = np.zeros(max(nq, 1), np.uint32)
+        lat = np.zeros(max(nq, 1), np.float64)
+        wall = C.c_double()
+        _check(_lib.ab_server_replay(self._h, _ptr(k), _ptr(off), nq, rate_qps, seed, _ptr(counts), _ptr(lat),
+                                     C.byref(wall)))
+        return counts[:nq], lat[:nq], wall.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for ctx in self.contexts:
+                ctx.close()
+            _lib.ab_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class SparseWorker:
     """Prepared sparse query (ivf_list::Prepare(keys), inverted_list.h:481)."""
 

@@ -636,14 +636,24 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
     const uint64_t out_words = a.out_words, out_stride = a.out_stride;
 
     // ---- stage qoff (+ host decisions, query lists) through pinned memory -----
-    // layout of hstage: qoff [nq+1 u64] | sel [nq u8, padded to 8] | sparse list [nq u32]
+    // layout of hstage: qoff [nq+1 u64] | sel [nq u8, padded to 8] | sparse list [nq u32, padded to 8]
+    //                   | id offsets [nq u64]
     AB_TRY(stage_wait(c));
     const size_t qoff_bytes = ((size_t)nq + 1) * 8;
     const size_t sel_off = qoff_bytes, list_off = qoff_bytes + ((size_t)nq + 7) / 8 * 8;
-    AB_TRY(c->hstage.ensure(list_off + (size_t)nq * 4 + 64));
+    const size_t idoff_off = list_off + ((size_t)nq * 4 + 7) / 8 * 8;
+    AB_TRY(c->hstage.ensure(idoff_off + (size_t)nq * 8 + 64));
     memcpy(c->hstage.p, a.h_qoff, qoff_bytes);
     AB_TRY(c->qoff.ensure(qoff_bytes));
     HIP_TRY(hipMemcpyAsync(c->qoff.p, c->hstage.p, qoff_bytes, hipMemcpyHostToDevice, s));
+    const uint64_t *d_idoff = nullptr;
+    if (a.h_id_off && a.d_ids) {
+        uint8_t *st = (uint8_t *)c->hstage.p + idoff_off;
+        memcpy(st, a.h_id_off, (size_t)nq * 8);
+        AB_TRY(c->idoff.ensure((size_t)nq * 8));
+        HIP_TRY(hipMemcpyAsync(c->idoff.p, st, (size_t)nq * 8, hipMemcpyHostToDevice, s));
+        d_idoff = (const uint64_t *)c->idoff.p;
+    }
     AB_TRY(c->sel.ensure(nq));
     AB_TRY(c->hsel.ensure(nq));
     AB_TRY(c->hM.ensure((size_t)nq * 8));
@@ -783,8 +793,8 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
             launch_popcount(bm, out_stride, out_words, nq, (uint32_t *)c->ccnt.p, s);
         }
         TimedSpan ts(c, AB_K_WRITE_IDS);
-        launch_write_ids(bm, out_stride, out_words, nq, (const uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride,
-                         a.d_counts, s);
+        launch_write_ids(bm, out_stride, out_words, nq, (const uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, d_idoff,
+                         a.id_off_aligned, a.d_counts, s);
     }
     HIP_TRY(hipGetLastError());
     // the staged control data stays in use until the copies above have run
@@ -953,21 +963,21 @@ extern "C" ab_status ab_list_union_sparse_batch(ab_ctx *c, const uint32_t *h_key
     return run_batch(c, a);
 }
 
-extern "C" ab_status ab_query_batch(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq, int path,
-                                    uint32_t *d_bitmaps, uint32_t *d_ids, uint64_t ids_stride, uint32_t *d_counts,
-                                    uint8_t *h_paths)
+static ab_status query_batch(const char *fn, ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq,
+                             int path, uint32_t *d_bitmaps, uint32_t *d_ids, uint64_t ids_stride,
+                             const uint64_t *h_id_off, uint32_t *d_counts, uint8_t *h_paths)
 {
     if (!c || !h_qoff || (h_qoff[nq] && !d_keys) || path < 0 || path > AB_PATH_LEARNED) {
-        set_error("ab_query_batch: bad argument");
+        set_error(std::string(fn) + ": bad argument");
         return AB_INVALID_ARG;
     }
-    if (d_ids && ids_stride == 0) {
-        set_error("ab_query_batch: ids_stride must be positive with d_ids");
+    if (d_ids && !h_id_off && ids_stride == 0) {
+        set_error(std::string(fn) + ": ids_stride must be positive with d_ids");
         return AB_INVALID_ARG;
     }
     for (uint32_t q = 0; q < nq; ++q)
         if (h_qoff[q + 1] < h_qoff[q]) {
-            set_error("ab_query_batch: offsets must be non-decreasing");
+            set_error(std::string(fn) + ": offsets must be non-decreasing");
             return AB_INVALID_ARG;
         }
     BatchArgs a;
@@ -981,10 +991,35 @@ extern "C" ab_status ab_query_batch(ab_ctx *c, const uint32_t *d_keys, const uin
     a.ids_stride = ids_stride;
     a.d_counts = d_counts;
     a.h_paths = h_paths;
+    if (h_id_off) {
+        a.h_id_off = h_id_off;
+        a.id_off_aligned = true;
+        for (uint32_t q = 0; q < nq; ++q) a.id_off_aligned = a.id_off_aligned && (h_id_off[q] % 4 == 0);
+    }
     // the fused path's grids put queries on the y dimension (at most 65535)
     // and its persistent kernel is built for 256 and 512 threads
     if (c->batch_mode == 0 || nq == 0 || nq > 65535 || c->dense_threads < 256) return run_batch(c, a);
     return run_batch_fused(c, a);
+}
+
+extern "C" ab_status ab_query_batch(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq, int path,
+                                    uint32_t *d_bitmaps, uint32_t *d_ids, uint64_t ids_stride, uint32_t *d_counts,
+                                    uint8_t *h_paths)
+{
+    return query_batch("ab_query_batch", c, d_keys, h_qoff, nq, path, d_bitmaps, d_ids, ids_stride, nullptr,
+                       d_counts, h_paths);
+}
+
+extern "C" ab_status ab_query_batch_packed(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq,
+                                           int path, uint32_t *d_bitmaps, uint32_t *d_ids,
+                                           const uint64_t *h_id_off, uint32_t *d_counts, uint8_t *h_paths)
+{
+    if (d_ids && !h_id_off) {
+        set_error("ab_query_batch_packed: h_id_off is required with d_ids");
+        return AB_INVALID_ARG;
+    }
+    return query_batch("ab_query_batch_packed", c, d_keys, h_qoff, nq, path, d_bitmaps, d_ids, 0,
+                       d_ids ? h_id_off : nullptr, d_counts, h_paths);
 }
 
 extern "C" ab_status ab_compact_ids(ab_ctx *c, const uint32_t *d_bitmaps, uint32_t nq, uint32_t *d_ids,
@@ -998,7 +1033,8 @@ extern "C" ab_status ab_compact_ids(ab_ctx *c, const uint32_t *d_bitmaps, uint32
     const uint64_t nchunk = (w + kCompactWords - 1) / kCompactWords;
     AB_TRY(c->ccnt.ensure((size_t)nq * nchunk * 4 + 4));
     launch_popcount(d_bitmaps, w, w, nq, (uint32_t *)c->ccnt.p, c->stream);
-    launch_write_ids(d_bitmaps, w, w, nq, (const uint32_t *)c->ccnt.p, d_ids, ids_stride, d_counts, c->stream);
+    launch_write_ids(d_bitmaps, w, w, nq, (const uint32_t *)c->ccnt.p, d_ids, ids_stride, nullptr, false, d_counts,
+                     c->stream);
     HIP_TRY(hipGetLastError());
     return AB_OK;
 }

@@ -40,6 +40,8 @@ struct BatchArgs {
     uint64_t out_words = 0;           // words written per query
     uint32_t *d_ids = nullptr;
     uint64_t ids_stride = 0;
+    const uint64_t *h_id_off = nullptr;  // per-query id offsets (host), replacing ids_stride when set
+    bool id_off_aligned = false;         // every offset is a multiple of 4
     uint32_t *d_counts = nullptr;
     uint8_t *h_paths = nullptr;
 };
@@ -77,7 +79,7 @@ struct ab_ctx {
     bool dense_columns = true;  // per-value query masks in dense groups when LDS allows
     uint32_t sparse_chunk = 4096;
     // device scratch
-    ab::DevBuf qoff, sel, kq, cnt, koff, parts, feat, M, ccnt, bm, qlist, keys, qbm, mask;
+    ab::DevBuf qoff, sel, kq, cnt, koff, parts, feat, M, ccnt, bm, qlist, keys, qbm, mask, idoff;
     // pinned staging: control data (qoff, decisions, query lists) and keys
     ab::PinBuf hstage, hkstage, hsel, hM;
     hipEvent_t stage_ev = nullptr, kstage_ev = nullptr;

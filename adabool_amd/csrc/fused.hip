@@ -52,6 +52,8 @@ struct Plan {
     int mode;
     bool use_col;
     uint32_t grid;  // work-groups of the persistent launch
+    bool packed;    // per-query id offsets follow qoff in the staged block
+    size_t stage_bytes;
     uint64_t mask_stride, any_stride;  // elements / words per group
     uint32_t *bm;
     uint64_t stride, words;
@@ -67,7 +69,8 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     const uint32_t nq = p.nq;
     const uint64_t *qoff = (const uint64_t *)c->qoff.p;
     uint8_t *sel = (uint8_t *)c->sel.p;
-    HIP_TRY(hipMemcpyAsync(c->qoff.p, c->hstage.p, ((size_t)nq + 1) * 8, hipMemcpyHostToDevice, s));
+    // qoff and, for packed ids, the id offsets arrive in one copy
+    HIP_TRY(hipMemcpyAsync(c->qoff.p, c->hstage.p, p.stage_bytes, hipMemcpyHostToDevice, s));
     {
         TimedSpan ts(c, AB_K_DECIDE);
         if (a.path == AB_PATH_DENSE || a.path == AB_PATH_SPARSE) {
@@ -132,8 +135,9 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
             launch_popcount(p.bm, p.stride, p.words, nq, (uint32_t *)c->ccnt.p, s);
         }
         TimedSpan ts(c, AB_K_WRITE_IDS);
-        launch_write_ids(p.bm, p.stride, p.words, nq, (const uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, a.d_counts,
-                         s);
+        const uint64_t *d_idoff = p.packed ? qoff + (nq + 1) : nullptr;
+        launch_write_ids(p.bm, p.stride, p.words, nq, (const uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, d_idoff,
+                         a.id_off_aligned, a.d_counts, s);
     }
     HIP_TRY(hipGetLastError());
     return AB_OK;
@@ -168,7 +172,9 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     for (DevBuf *b : {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan, &c->counter,
                       &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm})
         before.push_back(b->p);
-    AB_TRY(c->qoff.ensure(((size_t)nq + 1) * 8));
+    p.packed = a.h_id_off && a.d_ids;
+    p.stage_bytes = ((size_t)nq + 1) * 8 + (p.packed ? (size_t)nq * 8 : 0);
+    AB_TRY(c->qoff.ensure(p.stage_bytes));
     AB_TRY(c->sel.ensure(nq));
     AB_TRY(c->feat.ensure((size_t)nq * AB_NFEATURES * 4));
     AB_TRY(c->M.ensure((size_t)nq * 8));
@@ -207,8 +213,9 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
 
     // ---- stage the offsets (the graph's copy node reads this pinned area) ----
     AB_TRY(stage_wait(c));
-    AB_TRY(c->hstage.ensure(((size_t)nq + 1) * 8 + 64));
+    AB_TRY(c->hstage.ensure(p.stage_bytes + 64));
     memcpy(c->hstage.p, a.h_qoff, ((size_t)nq + 1) * 8);
+    if (p.packed) memcpy((uint8_t *)c->hstage.p + ((size_t)nq + 1) * 8, a.h_id_off, (size_t)nq * 8);
 
     const bool graph = c->batch_mode == 2 && !c->timing;
     if (!graph) {
@@ -216,7 +223,8 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     } else {
         std::vector<uint64_t> key = {(uint64_t)(uintptr_t)a.d_keys, nq, p.K, (uint64_t)a.path, p.grid,
                                      (uint64_t)(uintptr_t)p.bm, p.stride, p.words, (uint64_t)(uintptr_t)a.d_ids,
-                                     a.ids_stride, (uint64_t)(uintptr_t)a.d_counts, c->cfg_gen,
+                                     a.ids_stride, (uint64_t)p.packed, (uint64_t)a.id_off_aligned,
+                                     (uint64_t)(uintptr_t)a.d_counts, c->cfg_gen,
                                      (uint64_t)(uintptr_t)c->hstage.p, (uint64_t)(uintptr_t)c->gmask.p,
                                      (uint64_t)(uintptr_t)c->ganybm.p};
         for (void *q : after) key.push_back((uint64_t)(uintptr_t)q);

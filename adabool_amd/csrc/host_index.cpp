@@ -350,8 +350,7 @@ ab_status build_host_index(const uint32_t *keys, const uint32_t *vals, uint64_t 
 void host_features(const uint32_t *seg8, uint32_t nkeys, uint32_t nvalues, uint64_t R,
                    const uint32_t *keys, uint32_t n, float *f, uint64_t *m_out)
 {
-    uint64_t valid = 0, M = 0, mx = 0, bytes = 0, nv = 0, a1 = 0, bs = 0;
-    double sq = 0;
+    uint64_t valid = 0, M = 0, mx = 0, bytes = 0, nv = 0, a1 = 0, bs = 0, sq = 0;  // exact integer sums
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t k = keys[i];
         if (k >= nkeys) continue;
@@ -359,7 +358,7 @@ void host_features(const uint32_t *seg8, uint32_t nkeys, uint32_t nvalues, uint6
         uint64_t tot = r1[6] - r0[6];
         ++valid;
         M += tot;
-        sq += (double)tot * (double)tot;
+        sq += tot * tot;
         if (tot > mx) mx = tot;
         for (int t = 0; t < kTypes; ++t) bytes += (uint64_t)(r1[t] - r0[t]) * (4 + kPayloadBytes[t]);
         nv += r1[7] - r0[7];
@@ -368,7 +367,7 @@ void host_features(const uint32_t *seg8, uint32_t nkeys, uint32_t nvalues, uint6
     }
     double dv = valid ? (double)valid : 1.0;
     double mean = (double)M / dv;
-    double var = sq / dv - mean * mean;
+    double var = (double)sq / dv - mean * mean;
     if (var < 0) var = 0;
     double dM = M ? (double)M : 1.0;
     double nvd = nvalues ? (double)nvalues : 1.0;

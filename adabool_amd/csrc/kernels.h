@@ -93,9 +93,11 @@ hipError_t launch_union_fused(int mode, uint32_t threads, const DevIndex &ix, co
 constexpr uint32_t kCompactWords = 2048;
 void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
                      uint32_t *ccnt, hipStream_t s);
+// Query q's ids go to ids + id_off[q] (device offsets) or, with id_off NULL,
+// to ids + q * ids_stride; off_aligned: every id_off[q] is a multiple of 4.
 void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
-                      const uint32_t *ccnt, uint32_t *ids, uint64_t ids_stride, uint32_t *counts,
-                      hipStream_t s);
+                      const uint32_t *ccnt, uint32_t *ids, uint64_t ids_stride, const uint64_t *id_off,
+                      bool off_aligned, uint32_t *counts, hipStream_t s);
 void launch_features(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff, uint32_t nq,
                      uint64_t R, float *feats, uint64_t *M, hipStream_t s);
 void launch_decide_threshold(const uint64_t *M, uint32_t nq, double thr, uint8_t *sel, hipStream_t s);

@@ -1045,7 +1045,7 @@ template <bool ALIGNED>
 __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
                                                    uint32_t nchunk, const uint32_t *__restrict__ ccnt,
                                                    uint32_t *__restrict__ ids, uint64_t ids_stride,
-                                                   uint32_t *__restrict__ counts)
+                                                   const uint64_t *__restrict__ id_off, uint32_t *__restrict__ counts)
 {
     __shared__ __attribute__((aligned(16))) uint32_t stage[4][kStage];
     __shared__ uint32_t wcnt[4];
@@ -1074,7 +1074,7 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
     if (!ids) return;
     uint64_t off = prefix;
     for (uint32_t w = 0; w < wave; ++w) off += wcnt[w];
-    uint32_t *dst = ids + (uint64_t)q * ids_stride;
+    uint32_t *dst = ids + (id_off ? id_off[q] : (uint64_t)q * ids_stride);
     uint32_t *stg = stage[wave];
 #pragma unroll 1
     for (int i = 0; i < 8; ++i) {
@@ -1119,62 +1119,98 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
 }
 
 void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, const uint32_t *ccnt,
-                      uint32_t *ids, uint64_t ids_stride, uint32_t *counts, hipStream_t s)
+                      uint32_t *ids, uint64_t ids_stride, const uint64_t *id_off, bool off_aligned,
+                      uint32_t *counts, hipStream_t s)
 {
     if (!nq || !words) return;
     uint32_t nchunk = (uint32_t)((words + kCompactWords - 1) / kCompactWords);
-    const bool aligned = ((uintptr_t)ids % 16 == 0) && (ids_stride % 4 == 0);
+    const bool aligned = ((uintptr_t)ids % 16 == 0) && (id_off ? off_aligned : ids_stride % 4 == 0);
     if (aligned)
         hipLaunchKernelGGL(k_write_ids<true>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt, ids,
-                           ids_stride, counts);
+                           ids_stride, id_off, counts);
     else
         hipLaunchKernelGGL(k_write_ids<false>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt,
-                           ids, ids_stride, counts);
+                           ids, ids_stride, id_off, counts);
 }
 
 // ---------------------------------------------------------------------------
 // cost-model features and the threshold rule
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_features(DevIndex ix, const uint32_t *__restrict__ keys,
-                                                 const uint64_t *__restrict__ qoff, uint64_t R, float *feats,
-                                                 uint64_t *Mout)
+// One 256-thread work-group per query; every thread keeps four keys' row
+// loads in flight.  All sums are integers (exact in any order), so the
+// features equal the host's bit for bit up to the final float conversion.
+constexpr uint32_t kFeatNT = 256;
+__global__ __launch_bounds__(kFeatNT) void k_features(DevIndex ix, const uint32_t *__restrict__ keys,
+                                                      const uint64_t *__restrict__ qoff, uint64_t R, float *feats,
+                                                      uint64_t *Mout)
 {
     constexpr uint32_t kBytes[6] = {4, 8, 12, 20, 36, 36};
-    const uint32_t q = blockIdx.x, lane = lane_id();
+    constexpr uint32_t U = 4;
+    __shared__ unsigned long long sh[kFeatNT / 64][8];
+    const uint32_t q = blockIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t a = qoff[q], e = qoff[q + 1];
-    unsigned long long valid = 0, M = 0, mx = 0, bytes = 0, nv = 0, a1 = 0, bs = 0;
-    double sq = 0;
-    for (uint64_t j = a + lane; j < e; j += 64) {
-        const uint32_t k = keys[j];
-        if (k >= ix.nkeys) continue;
-        const uint32_t *r0 = ix.seg8 + (uint64_t)k * 8, *r1 = r0 + 8;
-        const unsigned long long tot = r1[6] - r0[6];
-        ++valid;
-        M += tot;
-        sq += (double)tot * (double)tot;
-        mx = tot > mx ? tot : mx;
-        for (int t = 0; t < 6; ++t) bytes += (unsigned long long)(r1[t] - r0[t]) * kBytes[t];
-        nv += r1[7] - r0[7];
-        a1 += r1[0] - r0[0];
-        bs += r1[5] - r0[5];
-    }
-    valid = wave_sum64(valid);
-    M = wave_sum64(M);
-    bytes = wave_sum64(bytes);
-    nv = wave_sum64(nv);
-    a1 = wave_sum64(a1);
-    bs = wave_sum64(bs);
+    unsigned long long v[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // valid M sq mx bytes nv a1 bs
+    for (uint64_t j0 = a + threadIdx.x; j0 < e; j0 += (uint64_t)kFeatNT * U) {
+        uint32_t k[U];
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        sq += __shfl_xor(sq, d, 64);
-        unsigned long long o = __shfl_xor(mx, d, 64);
-        mx = o > mx ? o : mx;
+        for (uint32_t u = 0; u < U; ++u) {
+            const uint64_t j = j0 + (uint64_t)u * kFeatNT;
+            k[u] = j < e ? keys[j] : 0xffffffffu;
+        }
+        uint4 lo0[U], lo1[U], hi0[U], hi1[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (k[u] < ix.nkeys) {
+                const uint4 *r = reinterpret_cast<const uint4 *>(ix.seg8 + (uint64_t)k[u] * 8);
+                lo0[u] = r[0];
+                lo1[u] = r[1];
+                hi0[u] = r[2];
+                hi1[u] = r[3];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            if (k[u] >= ix.nkeys) continue;
+            const uint32_t c[8] = {hi0[u].x - lo0[u].x, hi0[u].y - lo0[u].y, hi0[u].z - lo0[u].z,
+                                   hi0[u].w - lo0[u].w, hi1[u].x - lo1[u].x, hi1[u].y - lo1[u].y,
+                                   hi1[u].z - lo1[u].z, hi1[u].w - lo1[u].w};
+            const unsigned long long tot = c[6];
+            v[0] += 1;
+            v[1] += tot;
+            v[2] += tot * tot;
+            v[3] = tot > v[3] ? tot : v[3];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) v[4] += (unsigned long long)c[t] * kBytes[t];
+            v[5] += c[7];
+            v[6] += c[0];
+            v[7] += c[5];
+        }
     }
-    if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i == 3) {
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) {
+                const unsigned long long o = __shfl_xor(v[3], d, 64);
+                v[3] = o > v[3] ? o : v[3];
+            }
+        } else {
+            v[i] = wave_sum64(v[i]);
+        }
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) sh[wave][i] = v[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t w = 0; w < kFeatNT / 64; ++w)
+            for (int i = 0; i < 8; ++i) t[i] = i == 3 ? (sh[w][i] > t[i] ? sh[w][i] : t[i]) : t[i] + sh[w][i];
+        const unsigned long long valid = t[0], M = t[1], mx = t[3], bytes = t[4], nv = t[5], a1 = t[6], bs = t[7];
         const double n = (double)(e - a);
         const double dv = valid ? (double)valid : 1.0;
         const double mean = (double)M / dv;
-        double var = sq / dv - mean * mean;
+        double var = (double)t[2] / dv - mean * mean;
         if (var < 0) var = 0;
         const double dM = M ? (double)M : 1.0;
         const double nvd = ix.nvalues ? (double)ix.nvalues : 1.0;
@@ -1200,7 +1236,7 @@ void launch_features(const DevIndex &ix, const uint32_t *keys, const uint64_t *q
                      float *feats, uint64_t *M, hipStream_t s)
 {
     if (!nq) return;
-    hipLaunchKernelGGL(k_features, dim3(nq), dim3(64), 0, s, ix, keys, qoff, R, feats, M);
+    hipLaunchKernelGGL(k_features, dim3(nq), dim3(kFeatNT), 0, s, ix, keys, qoff, R, feats, M);
 }
 
 __global__ void k_decide_threshold(const uint64_t *M, uint32_t nq, double thr, uint8_t *sel)

@@ -1,0 +1,94 @@
+// tune.hip -- launch-parameter autotuner: times every candidate setting of a
+// context on a sample batch and keeps the fastest.
+//
+// Knobs (all on the device-dispatched batch path):
+//   dense group width  queries that share one pass over the value-tiled key
+//                      stream (1..64); sets the LDS row-tile width (group x
+//                      tile bits) and the query-mask word (u32 / u64)
+//   dense threads      work-group size of the persistent kernel (256 / 512),
+//                      i.e. waves per value tile
+//   sparse chunk       records per sparse work item (256..8192)
+//   dense columns      per-value query masks in LDS (on / off)
+// The value tile itself is a layout parameter of the index (ab_index_build's
+// tile_shift); tuning it means building the index once per candidate.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+using namespace ab;
+
+extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq,
+                                     int path, uint32_t repeats, ab_tune_config *out, uint32_t cap,
+                                     uint32_t *n_out)
+{
+    if (!c || !h_qoff || nq == 0 || (h_qoff[nq] && !d_keys) || repeats == 0 || (cap && !out)) {
+        set_error("ab_ctx_autotune: bad argument");
+        return AB_INVALID_ARG;
+    }
+    const uint32_t groups[] = {8, 16, 32, 48, 64};
+    const uint32_t threads[] = {256, 512};
+    const uint32_t chunks[] = {1024, 4096};
+    const int columns[] = {1, 0};
+    std::vector<ab_tune_config> all;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        set_error("ab_ctx_autotune: hipEventCreate failed");
+        return AB_DEVICE_ERROR;
+    }
+    const uint32_t saved[4] = {c->group, c->dense_threads, c->sparse_chunk, (uint32_t)c->dense_columns};
+    ab_status st = AB_OK;
+    for (uint32_t g : groups)
+        for (uint32_t t : threads)
+            for (uint32_t ch : chunks)
+                for (int col : columns) {
+                    if (st != AB_OK) break;
+                    if ((st = ab_ctx_set_dense_group(c, g)) != AB_OK) break;
+                    if ((st = ab_ctx_set_dense_threads(c, t)) != AB_OK) break;
+                    if ((st = ab_ctx_set_sparse_chunk(c, ch)) != AB_OK) break;
+                    if ((st = ab_ctx_set_dense_columns(c, col)) != AB_OK) break;
+                    // warm-up (allocation, graph capture), then timed repeats
+                    st = ab_query_batch(c, d_keys, h_qoff, nq, path, nullptr, nullptr, 0, nullptr, nullptr);
+                    if (st != AB_OK) break;
+                    (void)hipEventRecord(e0, c->stream);
+                    for (uint32_t r = 0; r < repeats && st == AB_OK; ++r)
+                        st = ab_query_batch(c, d_keys, h_qoff, nq, path, nullptr, nullptr, 0, nullptr, nullptr);
+                    (void)hipEventRecord(e1, c->stream);
+                    float ms = 0;
+                    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                        set_error("ab_ctx_autotune: event timing failed");
+                        st = AB_DEVICE_ERROR;
+                    }
+                    ab_tune_config cfg{};
+                    cfg.group = max_group(c);  // the width actually used after the LDS limit
+                    cfg.dense_threads = t;
+                    cfg.sparse_chunk = ch;
+                    cfg.dense_columns = col && col_fits(c, group_mode(cfg.group), cfg.group);
+                    cfg.ms = (double)ms / repeats;
+                    all.push_back(cfg);
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != AB_OK || all.empty()) {
+        (void)ab_ctx_set_dense_group(c, saved[0]);
+        (void)ab_ctx_set_dense_threads(c, saved[1]);
+        (void)ab_ctx_set_sparse_chunk(c, saved[2]);
+        (void)ab_ctx_set_dense_columns(c, (int)saved[3]);
+        return st != AB_OK ? st : AB_DEVICE_ERROR;
+    }
+    std::stable_sort(all.begin(), all.end(), [](const ab_tune_config &a, const ab_tune_config &b) {
+        return a.ms < b.ms;
+    });
+    const ab_tune_config &best = all.front();
+    (void)ab_ctx_set_dense_group(c, best.group);
+    (void)ab_ctx_set_dense_threads(c, best.dense_threads);
+    (void)ab_ctx_set_sparse_chunk(c, best.sparse_chunk);
+    (void)ab_ctx_set_dense_columns(c, best.dense_columns);
+    const uint32_t n = std::min<uint32_t>(cap, (uint32_t)all.size());
+    for (uint32_t i = 0; i < n; ++i) out[i] = all[i];
+    if (n_out) *n_out = (uint32_t)all.size();
+    return AB_OK;
+}

@@ -163,6 +163,26 @@ ab_status ab_ctx_set_sparse_chunk(ab_ctx *ctx, uint32_t items);
  * runs mode 2 batches as mode 1. */
 ab_status ab_ctx_set_batch_mode(ab_ctx *ctx, int mode);
 
+/* Autotuner.  Times ab_query_batch on the given batch (device keys, host
+ * offsets; union only, no compaction) for every candidate of the grid
+ *   dense group width {8,16,32,48,64} x dense threads {256,512}
+ *   x sparse chunk {1024,4096} x dense columns {on,off},
+ * `repeats` calls each after one warm-up call, and leaves the context set to
+ * the fastest.  out (may be NULL) receives up to cap candidates, fastest
+ * first, with their mean ms per batch; n_out the number timed.  The value
+ * tile is an index layout parameter (ab_index_build's tile_shift): tune it
+ * by building one index per candidate.  No reference counterpart (its launch
+ * shapes are compile-time template arguments). */
+typedef struct ab_tune_config {
+    uint32_t group;          /* effective dense group width */
+    uint32_t dense_threads;
+    uint32_t sparse_chunk;
+    int dense_columns;       /* effective (on only where the LDS fits) */
+    double ms;               /* mean device ms per batch */
+} ab_tune_config;
+ab_status ab_ctx_autotune(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq, int path,
+                          uint32_t repeats, ab_tune_config *out, uint32_t cap, uint32_t *n_out);
+
 /* Kernel classes reported by ab_ctx_timing. */
 enum {
     AB_K_DECIDE = 0,      /* device features + dispatch decision */
@@ -239,6 +259,16 @@ ab_status ab_query_batch(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_
                          uint32_t nq, int path, uint32_t *d_bitmaps, uint32_t *d_ids,
                          uint64_t ids_stride, uint32_t *d_counts, uint8_t *h_paths);
 
+/* ab_query_batch with ids packed at caller-chosen offsets: query i's ids go
+ * to d_ids + h_id_off[i] (u32 units; ranges must not overlap, and offsets
+ * that are multiples of 4 take the 128-bit store path).  Variable-size id
+ * lists packed on the device, where the reference's batch caller reads every
+ * bitmap back and collects each query's ids on the host
+ * (test/adsfilter_batch.cu:505-537). */
+ab_status ab_query_batch_packed(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_qoff,
+                                uint32_t nq, int path, uint32_t *d_bitmaps, uint32_t *d_ids,
+                                const uint64_t *h_id_off, uint32_t *d_counts, uint8_t *h_paths);
+
 /* Bitmap -> ascending ids for nq bitmaps at d_bitmaps + i * stride
  * (CollectIdsFromBitmap, test/adsfilter_q_all.cu:108-119, on the device). */
 ab_status ab_compact_ids(ab_ctx *ctx, const uint32_t *d_bitmaps, uint32_t nq,
@@ -286,6 +316,65 @@ ab_status ab_mlp_grad(ab_ctx *ctx, int slot, const float *h_x, const float *h_y,
  * data.  loss_curve (may be NULL) receives the mean loss of each epoch. */
 ab_status ab_mlp_train(ab_ctx *ctx, int slot, const float *h_x, const float *h_y, uint32_t n,
                        uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *loss_curve);
+
+/* ---- serving loop ----------------------------------------------------------
+ * A resident index served to requests submitted from any host thread.  Each
+ * request reserves result memory for its id upper bound (sum of its keys'
+ * list lengths, capped at nvalues) in a device arena at admission, in arrival
+ * order; admitted requests are batched continuously into device-dispatched
+ * batches (ab_query_batch_packed) on `inflight` contexts, each with its own
+ * stream, so one batch's compaction overlaps the next one's union.  Results
+ * stay valid until ab_server_release.  Replaces the reference's harness loop
+ * over query files (test/adsfilter_batch.cu:428-568), which ran fixed
+ * batches synchronously. */
+typedef struct ab_server ab_server;
+typedef struct ab_server_config {
+    uint32_t max_batch;   /* queries per batch (1..4096), default 1024 */
+    uint32_t max_wait_us; /* a partial batch leaves after this wait when the device is busy (200) */
+    uint32_t inflight;    /* batches in flight, one context and stream each (1..8, default 2) */
+    uint64_t arena_bytes; /* device memory for result ids (default 4 GiB) */
+    int path;             /* dispatch of every request (default AB_PATH_AUTO) */
+    int deliver;          /* 0: ids in device memory; 1: ids copied to a pinned host mirror */
+    int batch_mode;       /* ab_ctx_set_batch_mode of the contexts (default 1) */
+} ab_server_config;
+typedef struct ab_result {
+    uint64_t ticket;
+    int status;           /* AB_OK, or AB_OOM when the request can never fit the arena */
+    int path;             /* path taken (AB_PATH_DENSE / AB_PATH_SPARSE) */
+    uint32_t count;       /* ids */
+    const uint32_t *ids;  /* device (deliver 0) or host (deliver 1) pointer, valid until release */
+    double latency_ms;    /* submit -> results ready */
+} ab_result;
+typedef struct ab_server_stats_t {
+    uint64_t completed, batches, dense_queries;
+    double mean_batch, qps;
+    double lat_mean_ms, lat_p50_ms, lat_p90_ms, lat_p99_ms, lat_max_ms;
+    uint64_t arena_used_bytes;
+} ab_server_stats_t;
+ab_status ab_server_config_default(ab_server_config *cfg);
+ab_status ab_server_create(ab_index *ix, const ab_server_config *cfg, ab_server **out);
+/* Context i (< inflight) of the server, for settings (thresholds, cost
+ * models, dense group...) made before the first submission. */
+ab_ctx *ab_server_ctx(ab_server *srv, uint32_t i);
+/* Thread-safe; the keys are copied. */
+ab_status ab_server_submit(ab_server *srv, const uint32_t *h_keys, uint32_t n, uint64_t *ticket);
+/* Up to cap completed requests in completion order; waits up to wait_ms
+ * (-1: until one completes, 0: no wait) when none is ready. */
+ab_status ab_server_poll(ab_server *srv, ab_result *out, uint32_t cap, uint32_t *n, int wait_ms);
+/* Returns a completed request's result memory to the arena. */
+ab_status ab_server_release(ab_server *srv, uint64_t ticket);
+/* Latency percentiles (CalculateStats definition) and throughput since the
+ * last reset, over requests returned by ab_server_poll. */
+ab_status ab_server_stats(ab_server *srv, ab_server_stats_t *out, int reset);
+/* Stops after finishing every submitted request; frees all memory. */
+ab_status ab_server_free(ab_server *srv);
+/* Open-loop load: query i of (h_keys, h_qoff) arrives at Poisson times of
+ * rate_qps (0: all at once); results are collected and released as they
+ * complete.  h_counts / h_lat_ms (may be NULL) get each query's id count and
+ * latency from its scheduled arrival; wall_s the time to the last result. */
+ab_status ab_server_replay(ab_server *srv, const uint32_t *h_keys, const uint64_t *h_qoff, uint32_t nq,
+                           double rate_qps, uint64_t seed, uint32_t *h_counts, double *h_lat_ms,
+                           double *wall_s);
 
 /* ---- synthetic data ------------------------------------------------------- */
 typedef struct ab_dataset ab_dataset;
