@@ -555,9 +555,16 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
             }
             use_col = nvals <= nw * pc;
         }
+        // words are taken from the front of a rotating copy: no run-time
+        // register indexing (which the compiler would put in scratch memory)
+        uint32_t ww[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ww[i] = w[i];
 #pragma unroll 1
         for (uint32_t j = 0; j < 8; ++j) {
-            uint32_t x = pick<8>(w, j);
+            uint32_t x = ww[0];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) ww[i] = ww[i + 1];
             if (!x) continue;
             if constexpr (MODE != kModeBitmap) {
                 if (use_col) {
@@ -588,9 +595,14 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
         uint32_t cw = h >> 5, bits = 1u << (h & 31);
         if (use_col) put(h);
         bool stop = false;
+        uint32_t ww[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ww[i] = w[i];
 #pragma unroll 1
         for (uint32_t j = 0; j < (uint32_t)(P / 4) && !stop; ++j) {
-            const uint32_t word = pick<P / 4>(w, j);
+            const uint32_t word = ww[0];
+#pragma unroll
+            for (int i = 0; i < P / 4 - 1; ++i) ww[i] = ww[i + 1];
 #pragma unroll
             for (uint32_t k = 0; k < 4; ++k) {
                 const uint32_t b = (word >> (8 * k)) & 0xffu;
@@ -616,7 +628,10 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
 
 // One 1024-record block of type T whose absolute keys are staged in kb
 // (record s*64 + lane at kb[s*64 + lane]).  Slots go four at a time: the four
-// probes, then the four header/payload loads, are issued before any emit.
+// any-bit probes first; then, for every slot whose key some query of the
+// group holds, the query-mask load and the header/payload loads are issued
+// together (the any-bit is exactly "mask != 0"); then the four emits, taking
+// each slot from the front of a rotating register set.
 template <int MODE, int T>
 __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *kb, uint32_t nv, uint32_t rec_base,
                                             const void *__restrict__ mask, const uint32_t *__restrict__ anybm,
@@ -628,20 +643,27 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
     const uint8_t *__restrict__ pay = ix.vpay[T];
 #pragma unroll 1
     for (uint32_t s0 = 0; s0 < kSlots; s0 += 4) {
-        M m[4];
+        uint32_t key[4], hit[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const uint32_t r = (s0 + j) * 64 + lane;
-            m[j] = r < nv ? probe<MODE>(mask, anybm, kb[r]) : (M)0;
+            key[j] = r < nv ? kb[r] : 0u;
+            hit[j] = r < nv ? (anybm[key[j] >> 5] >> (key[j] & 31)) & 1u : 0u;
         }
-        if (!(m[0] | m[1] | m[2] | m[3])) continue;
+        if (!(hit[0] | hit[1] | hit[2] | hit[3])) continue;
+        M m[4];
         uint32_t h[4], w[4][8];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
+            m[j] = 0;
             h[j] = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i) w[j][i] = 0;
-            if (m[j]) {
+            if (hit[j]) {
+                if constexpr (MODE == kModeBitmap)
+                    m[j] = 1u;
+                else
+                    m[j] = reinterpret_cast<const M *>(mask)[key[j]];
                 const uint32_t idx = rec_base + (s0 + j) * 64 + lane;
                 h[j] = hdr[idx];
                 load_payload<T>(pay, idx, w[j]);
@@ -649,14 +671,15 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
         }
 #pragma unroll 1
         for (uint32_t j = 0; j < 4; ++j) {
-            const M mj = j == 0 ? m[0] : (j == 1 ? m[1] : (j == 2 ? m[2] : m[3]));
-            if (!mj) continue;
-            const uint32_t hj = j == 0 ? h[0] : (j == 1 ? h[1] : (j == 2 ? h[2] : h[3]));
-            uint32_t wj[8];
+            if (m[0]) dense_emit<MODE, T>(h[0], w[0], m[0], tiles, tw, tile_word0, col, vmask);
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-                wj[i] = j == 0 ? w[0][i] : (j == 1 ? w[1][i] : (j == 2 ? w[2][i] : w[3][i]));
-            dense_emit<MODE, T>(hj, wj, mj, tiles, tw, tile_word0, col, vmask);
+            for (int q = 0; q < 3; ++q) {
+                m[q] = m[q + 1];
+                h[q] = h[q + 1];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) w[q][i] = w[q + 1][i];
+            }
+            m[3] = 0;
         }
     }
 }

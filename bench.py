@@ -81,10 +81,21 @@ def measured_traffic(path: str, kclass: str):
     return float(v["fetch_bytes_median"] + v["write_bytes_median"]), f"{os.path.relpath(path, ROOT)}: {name} median"
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float):
     """Oracle (plain C, CSR-by-key union -> bitmap + ids) on the host cores,
-    over a bounded prefix of the same queries; also checks the GPU counts of
-    that prefix against it."""
+    over a bounded prefix of the same queries (one thread per core across
+    queries, and one thread alone); also checks the GPU counts of that prefix
+    against it."""
     import oracle
 
     nthreads = min(16, os.cpu_count() or 1)
@@ -98,10 +109,16 @@ def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float
     sub = qoff[: ns + 1]
     t, counts, lat, _ = csr.union_batch(keys[: int(sub[-1])], sub, nthreads)
     ok = bool(np.array_equal(counts.astype(np.int64), counts_gpu[:ns].astype(np.int64)))
+    # one thread: a smaller prefix of about a quarter of the budget
+    n1 = int(min(nq, max(1, budget_s / 4 / (per_q * nthreads))))
+    t1, _, lat1, _ = csr.union_batch(keys[: int(qoff[n1])], qoff[: n1 + 1], 1)
     return {"value": ns / t, "unit": "queries/sec", "cores": nthreads, "kind": "port",
             "sample": f"first {ns} of the {nq} C2 queries, oracle/oracle.c or_union_batch, {nthreads} threads, "
-                      f"{t:.1f} s", "counts_match_gpu": ok,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+                      f"{t:.1f} s", "counts_match_gpu": ok, "cpu_model": cpu_model(),
+            "host_cpus_visible": os.cpu_count(),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "single_thread": {"value": n1 / t1, "queries": n1, "p50_ms": float(np.percentile(lat1, 50)),
+                              "p99_ms": float(np.percentile(lat1, 99))}}
 
 
 def main():
