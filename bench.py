@@ -143,6 +143,7 @@ def main():
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "c2_batch_step.json"),
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (roofline traffic)")
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -212,6 +213,15 @@ def main():
     d_ids = torch.empty(nq * nv, dtype=torch.int32, device="cuda")
     d_cnt = torch.empty(nq, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
+    tuned = None
+    explicit = args.dense_threads or args.group or args.no_columns
+    if not args.no_autotune and not explicit:
+        t = time.time()
+        res = ctx.autotune(d_keys, qo, path, repeats=2)
+        dflt = next((r["ms"] for r in res if r["group"] == 64 and r["dense_threads"] == 512
+                     and r["sparse_chunk"] == 4096 and r["dense_columns"]), None)
+        tuned = dict(res[0], default_ms=dflt, candidates=len(res))
+        log(f"autotune: {res[0]} of {len(res)} candidates (default {dflt}) in {time.time() - t:.1f}s")
 
     def run(p, steps, warmup):
         """warmup + timed steps of the batch on dispatch p (no per-kernel events)."""
@@ -350,6 +360,7 @@ def main():
         "per_query_threshold_tau": ctx.threshold,
         "batched_threshold": thr_cmp,
         "batch_modes_qps": mode_cmp,
+        "autotune": tuned,
         "cost_model": model_meta,
         "speedup_vs_per_query_threshold": round(qps / base_qps, 2),
         "job_algorithmic_GBps": round(job_gbs, 1),
