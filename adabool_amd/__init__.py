@@ -56,7 +56,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def lib_path() -> str:
-    return os.path.join(_HERE, "libadabool.so")
+    """The in-tree library, or the build named by ADABOOL_LIB (for comparing builds)."""
+    return os.environ.get("ADABOOL_LIB") or os.path.join(_HERE, "libadabool.so")
 
 
 class AdaBoolError(RuntimeError):
@@ -86,6 +87,8 @@ _SIGS = {
     "ab_index_build_host": (C.c_int, [_VP, _VP, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_VP)]),
     "ab_index_load_coo": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_VP)]),
     "ab_index_free": (C.c_int, [_VP]),
+    "ab_index_save": (C.c_int, [_VP, C.c_char_p]),
+    "ab_index_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_VP)]),
     "ab_index_get_info": (C.c_int, [_VP, C.POINTER(_IndexInfo)]),
     "ab_result_stride_bytes": (C.c_uint64, [_VP]),
     "ab_index_export_type": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP]),
@@ -328,6 +331,20 @@ class Index:
         obj._h = h
         obj.host_only = False
         return obj
+
+    @classmethod
+    def load(cls, path: str, host_only: bool = False) -> "Index":
+        """Index from a binary index file written by save()."""
+        load_library()
+        obj = cls.__new__(cls)
+        h = _VP()
+        _check(_lib.ab_index_load(os.fsencode(path), 1 if host_only else 0, C.byref(h)))
+        obj._h = h
+        obj.host_only = host_only
+        return obj
+
+    def save(self, path: str):
+        _check(_lib.ab_index_save(self._h, os.fsencode(path)))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -146,17 +146,10 @@ static void fill_meta(ab_index *ix, const HostIndex &h)
     ix->stride = stride_bytes(h.nvalues);
 }
 
-static ab_status do_build(const uint32_t *keys, const uint32_t *vals, uint64_t nnz, uint32_t nkeys,
-                          uint32_t nvalues, uint32_t tile_shift, bool host_only, ab_index **out)
+// Makes a built host layout resident (or keeps it on the host for host-only
+// handles).
+static ab_status finish_index(std::unique_ptr<ab_index> &ix, bool host_only)
 {
-    if (!out) {
-        set_error("ab_index_build: NULL out");
-        return AB_INVALID_ARG;
-    }
-    *out = nullptr;
-    std::unique_ptr<ab_index> ix(new ab_index());
-    ab_status st = build_host_index(keys, vals, nnz, nkeys, nvalues, tile_shift, 0, &ix->host);
-    if (st != AB_OK) return st;
     fill_meta(ix.get(), ix->host);
     ix->seg8 = ix->host.seg8;
     ix->host_only = host_only;
@@ -185,6 +178,214 @@ static ab_status do_build(const uint32_t *keys, const uint32_t *vals, uint64_t n
         // only seg8 (already copied to ix->seg8) is needed on the host once resident
         h = HostIndex();
     }
+    return AB_OK;
+}
+
+static ab_status do_build(const uint32_t *keys, const uint32_t *vals, uint64_t nnz, uint32_t nkeys,
+                          uint32_t nvalues, uint32_t tile_shift, bool host_only, ab_index **out)
+{
+    if (!out) {
+        set_error("ab_index_build: NULL out");
+        return AB_INVALID_ARG;
+    }
+    *out = nullptr;
+    std::unique_ptr<ab_index> ix(new ab_index());
+    ab_status st = build_host_index(keys, vals, nnz, nkeys, nvalues, tile_shift, 0, &ix->host);
+    if (st != AB_OK) return st;
+    AB_TRY(finish_index(ix, host_only));
+    *out = ix.release();
+    return AB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// binary index file: a header, then every array of both layouts in a fixed
+// order (each as u64 byte count + bytes, padded to 8), then a 64-bit FNV-1a
+// checksum of everything before it.  Little-endian, as the device.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr char kMagic[8] = {'A', 'B', 'I', 'D', 'X', '0', '0', '1'};
+struct FileHeader {
+    char magic[8];
+    uint32_t version, nkeys, nvalues, tile_shift, ntiles, nsections;
+    uint64_t nnz;
+};
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void *p, size_t n)
+    {
+        const uint8_t *b = (const uint8_t *)p;
+        for (size_t i = 0; i < n; ++i) {
+            h ^= b[i];
+            h *= 1099511628211ull;
+        }
+    }
+};
+
+// The arrays of a layout, in file order.
+template <class F> void for_sections(HostIndex &h, F f)
+{
+    f(h.seg8);
+    for (int t = 0; t < kTypes; ++t) {
+        f(h.hdr[t]);
+        f(h.pay[t]);
+        f(h.vhdr[t]);
+        f(h.vpay[t]);
+    }
+    f(h.vblk);
+    f(h.vkeys);
+    f(h.tile_blk);
+}
+
+template <class T> ab_status download(const T *d, size_t n, std::vector<T> &v)
+{
+    v.resize(n);
+    if (n) HIP_TRY(hipMemcpy(v.data(), d, n * sizeof(T), hipMemcpyDeviceToHost));
+    return AB_OK;
+}
+}  // namespace
+
+extern "C" ab_status ab_index_save(const ab_index *ix, const char *path)
+{
+    if (!ix || !path) {
+        set_error("ab_index_save: NULL argument");
+        return AB_INVALID_ARG;
+    }
+    HostIndex copy;
+    HostIndex *h = nullptr;
+    if (ix->host_only) {
+        h = const_cast<HostIndex *>(&ix->host);
+    } else {
+        // the resident layout is the only copy: bring it back
+        const DevIndex &d = ix->dev;
+        copy.nkeys = ix->nkeys;
+        copy.nvalues = ix->nvalues;
+        copy.nnz = ix->nnz;
+        copy.tile_shift = ix->tile_shift;
+        copy.ntiles = ix->ntiles;
+        copy.seg8 = ix->seg8;
+        for (int t = 0; t < kTypes; ++t) {
+            AB_TRY(download(d.hdr[t], ix->nrec[t], copy.hdr[t]));
+            AB_TRY(download(d.pay[t], ix->nrec[t] * kPayloadBytes[t], copy.pay[t]));
+            AB_TRY(download(d.vhdr[t], ix->nrec[t], copy.vhdr[t]));
+            AB_TRY(download(d.vpay[t], ix->nrec[t] * kPayloadBytes[t], copy.vpay[t]));
+        }
+        std::vector<uint4> blk;
+        AB_TRY(download(d.vblk, ix->nblocks, blk));
+        copy.vblk.resize(blk.size());
+        if (!blk.empty()) memcpy(copy.vblk.data(), blk.data(), blk.size() * sizeof(uint4));
+        AB_TRY(download(d.vkeys, ix->kbytes + 64, copy.vkeys));
+        AB_TRY(download(d.tile_blk, (size_t)ix->ntiles + 1, copy.tile_blk));
+        h = &copy;
+    }
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("ab_index_save: cannot create ") + path);
+        return AB_IO_ERROR;
+    }
+    Fnv sum;
+    bool ok = true;
+    auto put = [&](const void *p, size_t n) {
+        if (n && fwrite(p, 1, n, f) != n) ok = false;
+        sum.add(p, n);
+    };
+    FileHeader hd{};
+    memcpy(hd.magic, kMagic, 8);
+    hd.version = 1;
+    hd.nkeys = h->nkeys;
+    hd.nvalues = h->nvalues;
+    hd.tile_shift = h->tile_shift;
+    hd.ntiles = h->ntiles;
+    hd.nsections = 1 + 4 * kTypes + 3;
+    hd.nnz = h->nnz;
+    put(&hd, sizeof(hd));
+    const uint8_t zeros[8] = {0};
+    for_sections(*h, [&](auto &v) {
+        const uint64_t bytes = v.size() * sizeof(v[0]);
+        put(&bytes, 8);
+        put(v.data(), bytes);
+        put(zeros, (8 - bytes % 8) % 8);
+    });
+    const uint64_t c = sum.h;
+    if (ok && fwrite(&c, 8, 1, f) != 1) ok = false;
+    if (fclose(f) != 0) ok = false;
+    if (!ok) {
+        set_error(std::string("ab_index_save: write failed: ") + path);
+        return AB_IO_ERROR;
+    }
+    return AB_OK;
+}
+
+extern "C" ab_status ab_index_load(const char *path, int host_only, ab_index **out)
+{
+    if (!path || !out) {
+        set_error("ab_index_load: NULL argument");
+        return AB_INVALID_ARG;
+    }
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_error(std::string("ab_index_load: cannot open ") + path);
+        return AB_IO_ERROR;
+    }
+    std::unique_ptr<ab_index> ix(new ab_index());
+    HostIndex &h = ix->host;
+    Fnv sum;
+    bool ok = true;
+    auto get = [&](void *p, size_t n) {
+        if (n && fread(p, 1, n, f) != n) ok = false;
+        if (ok) sum.add(p, n);
+    };
+    FileHeader hd{};
+    get(&hd, sizeof(hd));
+    if (!ok || memcmp(hd.magic, kMagic, 8) != 0 || hd.version != 1 || hd.nsections != 1 + 4 * kTypes + 3) {
+        fclose(f);
+        set_error(std::string("ab_index_load: not an index file of this version: ") + path);
+        return AB_IO_ERROR;
+    }
+    h.nkeys = hd.nkeys;
+    h.nvalues = hd.nvalues;
+    h.nnz = hd.nnz;
+    h.tile_shift = hd.tile_shift;
+    h.ntiles = hd.ntiles;
+    // section sizes are bounded by what is left of the file
+    long here = ftell(f);
+    fseek(f, 0, SEEK_END);
+    const uint64_t fsize = (uint64_t)ftell(f);
+    fseek(f, here, SEEK_SET);
+    uint8_t pad[8];
+    for_sections(h, [&](auto &v) {
+        if (!ok) return;
+        uint64_t bytes = 0;
+        get(&bytes, 8);
+        if (!ok || bytes % sizeof(v[0]) != 0 || bytes > fsize) {
+            ok = false;
+            return;
+        }
+        v.resize(bytes / sizeof(v[0]));
+        get(v.data(), bytes);
+        get(pad, (8 - bytes % 8) % 8);
+    });
+    uint64_t c = 0;
+    const uint64_t want = sum.h;
+    if (ok && fread(&c, 8, 1, f) != 1) ok = false;
+    fclose(f);
+    if (!ok || c != want) {
+        set_error(std::string("ab_index_load: truncated or corrupt file: ") + path);
+        return AB_IO_ERROR;
+    }
+    // structural checks the kernels rely on
+    bool good = h.seg8.size() == ((size_t)h.nkeys + 1) * kSegStride && h.tile_blk.size() == (size_t)h.ntiles + 1 &&
+                h.vkeys.size() >= 64;
+    for (int t = 0; t < kTypes && good; ++t)
+        good = h.pay[t].size() == h.hdr[t].size() * kPayloadBytes[t] && h.vhdr[t].size() == h.hdr[t].size() &&
+               h.vpay[t].size() == h.pay[t].size() && h.seg8[(size_t)h.nkeys * kSegStride + t] == h.hdr[t].size();
+    if (good && !h.tile_blk.empty()) good = h.tile_blk.back() == h.vblk.size();
+    if (!good) {
+        set_error(std::string("ab_index_load: inconsistent index file: ") + path);
+        return AB_IO_ERROR;
+    }
+    AB_TRY(finish_index(ix, host_only != 0));
     *out = ix.release();
     return AB_OK;
 }
@@ -203,15 +404,13 @@ extern "C" ab_status ab_index_build_host(const uint32_t *keys, const uint32_t *v
 
 extern "C" ab_status ab_index_load_coo(const char *path, uint32_t tile_shift, ab_index **out)
 {
+    std::vector<uint32_t> k, v;
     uint32_t mk = 0, mv = 0;
-    int64_t n = ab_read_coo(path, nullptr, nullptr, 0, &mk, &mv);
-    if (n < 0) return AB_IO_ERROR;
-    std::vector<uint32_t> k((size_t)n), v((size_t)n);
-    if (ab_read_coo(path, k.data(), v.data(), (uint64_t)n, &mk, &mv) != n) {
-        set_error("ab_index_load_coo: file changed while reading");
+    if (!read_coo_file(path, k, v, mk, mv)) {
+        set_error(std::string("ab_index_load_coo: cannot open ") + (path ? path : "(null)"));
         return AB_IO_ERROR;
     }
-    return ab_index_build(k.data(), v.data(), (uint64_t)n, 0, 0, tile_shift, out);
+    return ab_index_build(k.data(), v.data(), (uint64_t)k.size(), 0, 0, tile_shift, out);
 }
 
 extern "C" ab_status ab_index_free(ab_index *ix)
@@ -411,7 +610,7 @@ ab_ctx::~ab_ctx()
     release_graph(this);
     for (DevBuf *b : {&qoff, &sel, &kq, &cnt, &koff, &parts, &feat, &M, &ccnt, &bm, &qlist, &keys, &qbm, &mask,
                       &mlp[0].params, &mlp[0].norm, &mlp[1].params, &mlp[1].norm, &qgroup, &grp_q, &grp_n, &plan,
-                      &counter, &gmask, &ganybm})
+                      &counter, &gmask, &ganybm, &idoff})
         b->release();
     for (PinBuf *b : {&hstage, &hkstage, &hsel, &hM})
         b->release();

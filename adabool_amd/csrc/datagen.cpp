@@ -249,9 +249,8 @@ inline bool num(const char *&p, const char *e, uint32_t &out)
 // Calls f(line_begin, line_end) for every trimmed line that is not blank or a
 // '#' / '%' comment.
 template <class F>
-void for_lines(const std::string &s, F &&f)
+void for_lines(const char *p, const char *end, F &&f)
 {
-    const char *p = s.data(), *end = p + s.size();
     while (p < end) {
         const char *e = (const char *)memchr(p, '\n', end - p);
         if (!e) e = end;
@@ -262,7 +261,71 @@ void for_lines(const std::string &s, F &&f)
         p = e + 1;
     }
 }
+template <class F>
+void for_lines(const std::string &s, F &&f)
+{
+    for_lines(s.data(), s.data() + s.size(), f);
+}
+
+// ReadCOO over the whole text with one thread per core: the text is cut at
+// line starts into equal chunks, each parsed into its own arrays, and the
+// chunks are joined in file order.
+void parse_coo(const std::string &s, std::vector<uint32_t> &keys, std::vector<uint32_t> &vals, uint32_t &mk,
+               uint32_t &mv)
+{
+    int T = s.size() < (1u << 22) ? 1 : default_threads();
+    std::vector<size_t> cut(T + 1, s.size());
+    cut[0] = 0;
+    for (int t = 1; t < T; ++t) {
+        size_t pos = std::max(cut[t - 1], s.size() / T * t);
+        const char *nl = pos < s.size() ? (const char *)memchr(s.data() + pos, '\n', s.size() - pos) : nullptr;
+        cut[t] = nl ? (size_t)(nl - s.data()) + 1 : s.size();
+    }
+    std::vector<std::vector<uint32_t>> tk(T), tv(T);
+    std::vector<uint32_t> tmk(T, 0), tmv(T, 0);
+    parallel_ranges(T, T, [&](uint64_t b, uint64_t e, int) {
+        for (uint64_t t = b; t < e; ++t) {
+            tk[t].reserve((cut[t + 1] - cut[t]) / 8 + 16);
+            tv[t].reserve((cut[t + 1] - cut[t]) / 8 + 16);
+            for_lines(s.data() + cut[t], s.data() + cut[t + 1], [&](const char *a, const char *z) {
+                uint32_t k, v;
+                if (!num(a, z, k) || !num(a, z, v)) return;
+                tk[t].push_back(k);
+                tv[t].push_back(v);
+                tmk[t] = std::max(tmk[t], k);
+                tmv[t] = std::max(tmv[t], v);
+            });
+        }
+    });
+    size_t n = 0;
+    for (int t = 0; t < T; ++t) n += tk[t].size();
+    keys.resize(n);
+    vals.resize(n);
+    size_t o = 0;
+    mk = mv = 0;
+    for (int t = 0; t < T; ++t) {
+        if (!tk[t].empty()) {
+            memcpy(keys.data() + o, tk[t].data(), tk[t].size() * 4);
+            memcpy(vals.data() + o, tv[t].data(), tv[t].size() * 4);
+        }
+        o += tk[t].size();
+        mk = std::max(mk, tmk[t]);
+        mv = std::max(mv, tmv[t]);
+    }
+}
 }  // namespace
+
+namespace ab {
+// Parses a COO text file (ReadCOO semantics) in parallel; false if unreadable.
+bool read_coo_file(const char *path, std::vector<uint32_t> &keys, std::vector<uint32_t> &vals, uint32_t &mk,
+                   uint32_t &mv)
+{
+    std::string s;
+    if (!path || !read_file(path, s)) return false;
+    parse_coo(s, keys, vals, mk, mv);
+    return true;
+}
+}  // namespace ab
 
 extern "C" int64_t ab_read_query(const char *path, uint32_t *keys, uint64_t cap)
 {
@@ -284,27 +347,22 @@ extern "C" int64_t ab_read_query(const char *path, uint32_t *keys, uint64_t cap)
 extern "C" int64_t ab_read_coo(const char *path, uint32_t *keys, uint32_t *values, uint64_t cap,
                                uint32_t *max_key, uint32_t *max_value)
 {
-    std::string s;
-    if (!path || !read_file(path, s)) {
+    std::vector<uint32_t> k, v;
+    uint32_t mk = 0, mv = 0;
+    if (!read_coo_file(path, k, v, mk, mv)) {
         set_error(std::string("ab_read_coo: cannot open ") + (path ? path : "(null)"));
         return -1;
     }
-    int64_t n = 0;
-    uint32_t mk = 0, mv = 0;
-    for_lines(s, [&](const char *a, const char *b) {
-        uint32_t k, v;
-        if (!num(a, b, k) || !num(a, b, v)) return;
-        if (keys && values && (uint64_t)n < cap) {
-            keys[n] = k;
-            values[n] = v;
+    if (keys && values) {
+        const uint64_t n = std::min<uint64_t>(cap, k.size());
+        if (n) {
+            memcpy(keys, k.data(), n * 4);
+            memcpy(values, v.data(), n * 4);
         }
-        mk = std::max(mk, k);
-        mv = std::max(mv, v);
-        ++n;
-    });
+    }
     if (max_key) *max_key = mk;
     if (max_value) *max_value = mv;
-    return n;
+    return (int64_t)k.size();
 }
 
 // CalculateStats, test/adsfilter_q_all.cu:258-307.

@@ -76,6 +76,9 @@ void host_features(const uint32_t *seg8, uint32_t nkeys, uint32_t nvalues, uint6
 
 uint64_t stride_bytes(uint32_t nvalues);
 int default_threads();
+// COO text file -> pairs in file order (ReadCOO rules), parsed in parallel.
+bool read_coo_file(const char *path, std::vector<uint32_t> &keys, std::vector<uint32_t> &vals, uint32_t &max_key,
+                   uint32_t &max_value);
 
 // Parallel for over [0, n) split into contiguous ranges.
 template <class F>

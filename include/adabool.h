@@ -108,6 +108,14 @@ ab_status ab_index_build_host(const uint32_t *h_keys, const uint32_t *h_values, 
                               uint32_t nkeys, uint32_t nvalues, uint32_t tile_shift,
                               ab_index **out);
 
+/* Binary index file: both resident layouts as built (a header, every array,
+ * a 64-bit FNV-1a checksum).  Loading skips the build; host_only != 0 loads
+ * without any device allocation (as ab_index_build_host).  A truncated,
+ * corrupt or other-version file is AB_IO_ERROR.  The reference rebuilt its
+ * index from the COO text on every run (test/adsfilter_q_all.cu:340-357). */
+ab_status ab_index_save(const ab_index *ix, const char *path);
+ab_status ab_index_load(const char *path, int host_only, ab_index **out);
+
 ab_status ab_index_free(ab_index *ix);
 ab_status ab_index_get_info(const ab_index *ix, ab_index_info *out);
 uint64_t ab_result_stride_bytes(const ab_index *ix);

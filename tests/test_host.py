@@ -179,6 +179,33 @@ def test_text_loaders_match_oracle():
             ab.read_query(os.path.join(d, "missing.coo"))
 
 
+def test_large_coo_parallel_parse_matches_oracle():
+    """A text file above the parallel-parse size (chunks cut at line starts)
+    with comments, blank lines, CRLF, tabs and bad lines spread through it."""
+    from tests.test_oracle import COO_TEXT
+
+    rng = np.random.default_rng(8)
+    k = rng.integers(0, 2**32, size=400_000, dtype=np.uint64)
+    v = rng.integers(0, 5_000_000, size=400_000, dtype=np.uint64)
+    lines = [f"{a} {b}" for a, b in zip(k.tolist(), v.tolist())]
+    for i in range(0, len(lines), 997):
+        lines[i] = lines[i] + "\r"
+    for i in range(5, len(lines), 1009):
+        lines[i] = "# comment " + lines[i]
+    for i in range(7, len(lines), 1013):
+        lines[i] = "\t" + lines[i].replace(" ", "\t")
+    text = "%%MatrixMarket\n" + "\n\n".join(lines[:50]) + "\n" + "\n".join(lines[50:]) + "\n" + COO_TEXT
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "big.coo")
+        open(p, "w").write(text)
+        assert os.path.getsize(p) > (1 << 22)
+        got = ab.read_coo(p)
+        want = oracle.read_coo(p)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        assert got[2:] == want[2:]
+        assert got[0].size > 390_000
+
+
 def test_stats_match_oracle():
     rng = np.random.default_rng(2)
     for n in (1, 2, 9, 1000):
