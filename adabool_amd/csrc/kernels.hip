@@ -627,11 +627,13 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
 }
 
 // One 1024-record block of type T whose absolute keys are staged in kb
-// (record s*64 + lane at kb[s*64 + lane]).  Slots go four at a time: the four
-// any-bit probes first; then, for every slot whose key some query of the
-// group holds, the query-mask load and the header/payload loads are issued
-// together (the any-bit is exactly "mask != 0"); then the four emits, taking
-// each slot from the front of a rotating register set.
+// (record s*64 + lane at kb[s*64 + lane]).  Slots go RS at a time (eight for
+// the narrow types, whose round fits in few registers; four for the wide
+// ones): the RS any-bit probes first; then, for every slot whose key some
+// query of the group holds, the query-mask load and the header/payload loads
+// together (the any-bit is exactly "mask != 0"); then the RS emits -- unrolled
+// for ARRAY1, whose emit is a few instructions, else taken from the front of a
+// rotating register set (no run-time register indexing, so no scratch).
 template <int MODE, int T>
 __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *kb, uint32_t nv, uint32_t rec_base,
                                             const void *__restrict__ mask, const uint32_t *__restrict__ anybm,
@@ -639,22 +641,26 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
                                             typename MaskOf<MODE>::T *col, uint32_t vmask)
 {
     using M = typename MaskOf<MODE>::T;
+    constexpr int RS = (T == AB_ARRAY1 || T == AB_ARRAY4 || T == AB_ARRAY8) ? 8 : 4;
+    constexpr int PW = TypeTraits<T>::P / 4 > 0 ? TypeTraits<T>::P / 4 : 1;  // payload words kept per slot
     const uint32_t *__restrict__ hdr = ix.vhdr[T];
     const uint8_t *__restrict__ pay = ix.vpay[T];
 #pragma unroll 1
-    for (uint32_t s0 = 0; s0 < kSlots; s0 += 4) {
-        uint32_t key[4], hit[4];
+    for (uint32_t s0 = 0; s0 < kSlots; s0 += RS) {
+        uint32_t key[RS], hit[RS];
+        uint32_t any_hit = 0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < RS; ++j) {
             const uint32_t r = (s0 + j) * 64 + lane;
             key[j] = r < nv ? kb[r] : 0u;
             hit[j] = r < nv ? (anybm[key[j] >> 5] >> (key[j] & 31)) & 1u : 0u;
+            any_hit |= hit[j];
         }
-        if (!(hit[0] | hit[1] | hit[2] | hit[3])) continue;
-        M m[4];
-        uint32_t h[4], w[4][8];
+        if (!any_hit) continue;
+        M m[RS];
+        uint32_t h[RS], w[RS][8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < RS; ++j) {
             m[j] = 0;
             h[j] = 0;
 #pragma unroll
@@ -669,17 +675,23 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
                 load_payload<T>(pay, idx, w[j]);
             }
         }
+        if constexpr (T == AB_ARRAY1) {
+#pragma unroll
+            for (int j = 0; j < RS; ++j)
+                if (m[j]) dense_emit<MODE, T>(h[j], w[j], m[j], tiles, tw, tile_word0, col, vmask);
+        } else {
 #pragma unroll 1
-        for (uint32_t j = 0; j < 4; ++j) {
-            if (m[0]) dense_emit<MODE, T>(h[0], w[0], m[0], tiles, tw, tile_word0, col, vmask);
+            for (uint32_t j = 0; j < (uint32_t)RS; ++j) {
+                if (m[0]) dense_emit<MODE, T>(h[0], w[0], m[0], tiles, tw, tile_word0, col, vmask);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                m[q] = m[q + 1];
-                h[q] = h[q + 1];
+                for (int q = 0; q < RS - 1; ++q) {
+                    m[q] = m[q + 1];
+                    h[q] = h[q + 1];
 #pragma unroll
-                for (int i = 0; i < 8; ++i) w[q][i] = w[q + 1][i];
+                    for (int i = 0; i < PW; ++i) w[q][i] = w[q + 1][i];
+                }
+                m[RS - 1] = 0;
             }
-            m[3] = 0;
         }
     }
 }
