@@ -277,6 +277,17 @@ def main():
         log(f"batched ({dispatch}, mode {m}): {mode_cmp[str(m)]:.0f} q/s")
     ctx.set_batch_mode(args.batch_mode)
 
+    # ---- the device's write ceiling for this output: a plain fill of the ids buffer
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    d_ids.fill_(0)
+    torch.cuda.synchronize()
+    ev0.record()
+    d_ids.fill_(1)
+    ev1.record()
+    torch.cuda.synchronize()
+    fill_gbs = d_ids.numel() * 4 / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+    log(f"plain fill of the {d_ids.numel() * 4 / 1e9:.1f} GB ids buffer: {fill_gbs:.0f} GB/s")
+
     # ---- algorithmic bytes (per query: 4|q| + list bytes + bitmap + 4|R| + 4)
     lb = list_bytes_per_query(ix, qk, qo)
     nkeys_q = np.diff(qo).astype(np.float64)
@@ -364,6 +375,7 @@ def main():
         "cost_model": model_meta,
         "speedup_vs_per_query_threshold": round(qps / base_qps, 2),
         "job_algorithmic_GBps": round(job_gbs, 1),
+        "write_ceiling_fill_GBps": round(fill_gbs, 1),
         "kernel_ms_per_step": {k: round(v[0] / ksteps, 4) for k, v in kt.items() if v[1]},
         "roofline": roof,
         "cpu_baseline": cpu,
