@@ -672,8 +672,8 @@ extern "C" double ab_ctx_get_batch_threshold(const ab_ctx *c) { return c ? c->ta
 
 extern "C" ab_status ab_ctx_set_dense_group(ab_ctx *c, uint32_t w)
 {
-    if (!c || w < 1 || w > (uint32_t)kMaxGroup) {
-        set_error("ab_ctx_set_dense_group: width must be 1..64");
+    if (!c || w < 1 || w > kGroupSlots) {
+        set_error("ab_ctx_set_dense_group: width must be 1..128");
         return AB_INVALID_ARG;
     }
     c->group = w;
@@ -796,7 +796,10 @@ extern "C" ab_status ab_ctx_timing(ab_ctx *c, double *ms, uint64_t *n, int reset
 // ---------------------------------------------------------------------------
 // orchestration
 // ---------------------------------------------------------------------------
-int ab::group_mode(uint32_t B) { return B == 1 ? kModeBitmap : (B <= 32 ? kModeMask32 : kModeMask64); }
+int ab::group_mode(uint32_t B)
+{
+    return B == 1 ? kModeBitmap : (B <= 32 ? kModeMask32 : (B <= 64 ? kModeMask64 : kModeMask128));
+}
 
 constexpr uint32_t kLdsBudget = 65536;
 
@@ -955,7 +958,8 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
 
     // ---- dense queries: groups of up to 64 per key-stream pass ----------------
     if (!dq.empty()) {
-        const uint32_t G = max_group(c);
+        // host-dispatched groups carry at most 64 queries (one u64 mask word)
+        const uint32_t G = std::min<uint32_t>(max_group(c), kMaxGroup);
         for (size_t g0 = 0; g0 < dq.size(); g0 += G) {
             DenseGroup g;
             g.B = (uint32_t)std::min<size_t>(G, dq.size() - g0);

@@ -153,7 +153,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     p.nq = a.nq;
     p.K = a.h_qoff[a.nq];
     p.G = std::max<uint32_t>(1, max_group(c));
-    p.mode = p.G <= 32 ? kModeMask32 : kModeMask64;
+    p.mode = p.G <= 32 ? kModeMask32 : (p.G <= 64 ? kModeMask64 : kModeMask128);
     p.use_col = col_fits(c, p.mode, p.G);
     HIP_TRY(fused_grid(p.mode, c->dense_threads, ix->dev, p.G, p.use_col, &p.grid));
     c->fused_grid = p.grid;
@@ -164,7 +164,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     const uint64_t nkeys = ix->nkeys;
     p.mask_stride = (nkeys + 63) / 64 * 64;
     p.any_stride = (nkeys + 31) / 32 + 16;
-    const size_t esize = p.mode == kModeMask32 ? 4 : 8;
+    const size_t esize = p.mode == kModeMask32 ? 4 : (p.mode == kModeMask64 ? 8 : 16);
 
     // ---- every buffer exists before anything is enqueued (capture-safe) ----
     const uint64_t nchunk = (p.words + kCompactWords - 1) / kCompactWords;
@@ -179,7 +179,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     AB_TRY(c->feat.ensure((size_t)nq * AB_NFEATURES * 4));
     AB_TRY(c->M.ensure((size_t)nq * 8));
     AB_TRY(c->qgroup.ensure((size_t)nq * 4));
-    AB_TRY(c->grp_q.ensure((size_t)ngmax * 64 * 4));
+    AB_TRY(c->grp_q.ensure((size_t)ngmax * kGroupSlots * 4));
     AB_TRY(c->grp_n.ensure((size_t)ngmax * 4 + 4));
     AB_TRY(c->plan.ensure(16));
     AB_TRY(c->counter.ensure(16));

@@ -484,8 +484,53 @@ void launch_mask_from_bitmaps(const uint32_t *bitmaps, uint64_t len_u32, uint32_
 // ---------------------------------------------------------------------------
 // dense path: value-tiled bitmap probe, one work-group per value tile
 // ---------------------------------------------------------------------------
+// the query set of a key (or of a value, in the column accumulators): one bit
+// per query of the group
+struct M128 {
+    unsigned long long lo, hi;
+};
 template <int MODE> struct MaskOf { using T = uint32_t; };
 template <> struct MaskOf<kModeMask64> { using T = unsigned long long; };
+template <> struct MaskOf<kModeMask128> { using T = M128; };
+
+__device__ __forceinline__ bool any_bits(uint32_t m) { return m != 0; }
+__device__ __forceinline__ bool any_bits(unsigned long long m) { return m != 0; }
+__device__ __forceinline__ bool any_bits(const M128 &m) { return (m.lo | m.hi) != 0; }
+__device__ __forceinline__ uint32_t popc_mask(uint32_t m) { return (uint32_t)__popc(m); }
+__device__ __forceinline__ uint32_t popc_mask(unsigned long long m) { return (uint32_t)__popcll(m); }
+__device__ __forceinline__ uint32_t popc_mask(const M128 &m) { return (uint32_t)(__popcll(m.lo) + __popcll(m.hi)); }
+template <class F> __device__ __forceinline__ void for_bits(unsigned long long m, uint32_t base, F &&f)
+{
+    while (m) {
+        f(base + (uint32_t)__builtin_ctzll(m));
+        m &= m - 1;
+    }
+}
+template <class F> __device__ __forceinline__ void for_bits(uint32_t m, F &&f) { for_bits((unsigned long long)m, 0, f); }
+template <class F> __device__ __forceinline__ void for_bits(unsigned long long m, F &&f) { for_bits(m, 0, f); }
+template <class F> __device__ __forceinline__ void for_bits(const M128 &m, F &&f)
+{
+    for_bits(m.lo, 0, f);
+    for_bits(m.hi, 64, f);
+}
+__device__ __forceinline__ void or_mask(uint32_t *p, uint32_t m) { atomicOr(p, m); }
+__device__ __forceinline__ void or_mask(unsigned long long *p, unsigned long long m) { atomicOr(p, m); }
+__device__ __forceinline__ void or_mask(M128 *p, const M128 &m)
+{
+    if (m.lo) atomicOr(&p->lo, m.lo);
+    if (m.hi) atomicOr(&p->hi, m.hi);
+}
+// 64-query chunk c of a mask
+__device__ __forceinline__ unsigned long long mask_word(uint32_t m, uint32_t) { return m; }
+__device__ __forceinline__ unsigned long long mask_word(unsigned long long m, uint32_t) { return m; }
+__device__ __forceinline__ unsigned long long mask_word(const M128 &m, uint32_t c) { return c ? m.hi : m.lo; }
+// bit b of a mask in memory (global atomics)
+__device__ __forceinline__ void set_mask_bit(uint32_t *p, uint32_t b) { atomicOr(p, 1u << b); }
+__device__ __forceinline__ void set_mask_bit(unsigned long long *p, uint32_t b) { atomicOr(p, 1ull << b); }
+__device__ __forceinline__ void set_mask_bit(M128 *p, uint32_t b)
+{
+    atomicOr(b < 64 ? &p->lo : &p->hi, 1ull << (b & 63));
+}
 
 // Query bits of key k: the 1-bit-per-key bitmap answers alone for one query
 // and filters the per-key mask loads (8 bytes per key) for a group.
@@ -532,17 +577,12 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
         if constexpr (MODE == kModeBitmap) {
             atomicOr(&tiles[lw], bits);
         } else {
-            M mm = m;
-            while (mm) {
-                const uint32_t b = (uint32_t)__builtin_ctzll((unsigned long long)mm);
-                mm &= mm - 1;
-                atomicOr(&tiles[b * tw + lw], bits);
-            }
+            for_bits(m, [&](uint32_t b) { atomicOr(&tiles[b * tw + lw], bits); });
         }
     };
-    auto put = [&](uint32_t v) { atomicOr(&col[v & vmask], m); };
+    auto put = [&](uint32_t v) { or_mask(&col[v & vmask], m); };
     uint32_t pc = 1;
-    if constexpr (MODE != kModeBitmap) pc = col ? (uint32_t)__popcll((unsigned long long)m) : 0u;
+    if constexpr (MODE != kModeBitmap) pc = col ? popc_mask(m) : 0u;
     if constexpr (T == AB_BITSET) {
         const uint32_t base = h >> 5;
         bool use_col = false;
@@ -661,7 +701,7 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
         uint32_t h[RS], w[RS][8];
 #pragma unroll
         for (int j = 0; j < RS; ++j) {
-            m[j] = 0;
+            m[j] = M{};
             h[j] = 0;
 #pragma unroll
             for (int i = 0; i < 8; ++i) w[j][i] = 0;
@@ -678,11 +718,11 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
         if constexpr (T == AB_ARRAY1) {
 #pragma unroll
             for (int j = 0; j < RS; ++j)
-                if (m[j]) dense_emit<MODE, T>(h[j], w[j], m[j], tiles, tw, tile_word0, col, vmask);
+                if (any_bits(m[j])) dense_emit<MODE, T>(h[j], w[j], m[j], tiles, tw, tile_word0, col, vmask);
         } else {
 #pragma unroll 1
             for (uint32_t j = 0; j < (uint32_t)RS; ++j) {
-                if (m[0]) dense_emit<MODE, T>(h[0], w[0], m[0], tiles, tw, tile_word0, col, vmask);
+                if (any_bits(m[0])) dense_emit<MODE, T>(h[0], w[0], m[0], tiles, tw, tile_word0, col, vmask);
 #pragma unroll
                 for (int q = 0; q < RS - 1; ++q) {
                     m[q] = m[q + 1];
@@ -690,7 +730,7 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
 #pragma unroll
                     for (int i = 0; i < PW; ++i) w[q][i] = w[q + 1][i];
                 }
-                m[RS - 1] = 0;
+                m[RS - 1] = M{};
             }
         }
     }
@@ -715,7 +755,7 @@ __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, co
     M *col = (MODE != kModeBitmap && use_col) ? reinterpret_cast<M *>(tiles + ntw) : nullptr;
     for (uint32_t i = threadIdx.x; i < ntw; i += NT) tiles[i] = 0;
     if (col)
-        for (uint32_t i = threadIdx.x; i < tv; i += NT) col[i] = 0;
+        for (uint32_t i = threadIdx.x; i < tv; i += NT) col[i] = M{};
     __syncthreads();
 
     const uint32_t lane = lane_id();
@@ -766,17 +806,23 @@ __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, co
     if (col) {
         // columns -> rows: lane l holds the query set of value 64*gi + l; the
         // ballot of bit b over the wave is query b's 64 bits of those values
+        // (64 queries at a time)
         for (uint32_t gi = wave; gi < (tv >> 6); gi += NW) {
-            const unsigned long long qv = (unsigned long long)col[gi * 64 + lane];
-            if (__ballot(qv != 0) == 0) continue;
-            unsigned long long mine = 0;
-            for (uint32_t b = 0; b < B; ++b) {
-                const unsigned long long bal = __ballot((qv >> b) & 1ull);
-                if (lane == b) mine = bal;
-            }
-            if (lane < B) {
-                tiles[lane * tw + 2 * gi] |= (uint32_t)mine;
-                tiles[lane * tw + 2 * gi + 1] |= (uint32_t)(mine >> 32);
+            const M cv = col[gi * 64 + lane];
+            for (uint32_t c = 0; c * 64 < B; ++c) {
+                const unsigned long long qv = mask_word(cv, c);
+                if (__ballot(qv != 0) == 0) continue;
+                const uint32_t nb = B - c * 64 < 64 ? B - c * 64 : 64;
+                unsigned long long mine = 0;
+                for (uint32_t b = 0; b < nb; ++b) {
+                    const unsigned long long bal = __ballot((qv >> b) & 1ull);
+                    if (lane == b) mine = bal;
+                }
+                if (lane < nb) {
+                    uint32_t *row = tiles + (c * 64 + lane) * tw;
+                    row[2 * gi] |= (uint32_t)mine;
+                    row[2 * gi + 1] |= (uint32_t)(mine >> 32);
+                }
             }
         }
         __syncthreads();
@@ -810,8 +856,9 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
 
 uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t threads, bool use_col)
 {
-    const uint32_t col =
-        (mode == kModeBitmap || !use_col) ? 0u : (1u << tile_shift) * (mode == kModeMask32 ? 4u : 8u);
+    const uint32_t col = (mode == kModeBitmap || !use_col)
+                             ? 0u
+                             : (1u << tile_shift) * (mode == kModeMask32 ? 4u : (mode == kModeMask64 ? 8u : 16u));
     return (threads / 64) * kBlockRecs * 4u + B * (1u << (tile_shift - 5)) * 4u + col;
 }
 
@@ -849,7 +896,8 @@ hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, co
 // persistent union launch over dense (group, tile) items and sparse chunks
 // ---------------------------------------------------------------------------
 // Dense queries in query order get rank r -> group r / G, slot r % G; qgroup[q]
-// = group * 64 + slot (kNoGroup for sparse queries); plan = {groups, dense}.
+// = group * kGroupSlots + slot (kNoGroup for sparse queries); plan = {groups,
+// dense}.
 __global__ __launch_bounds__(1024) void k_assign_groups(const uint8_t *__restrict__ sel, uint32_t nq, uint32_t G,
                                                         uint32_t *qgroup, uint32_t *grp_q, uint32_t *grp_n,
                                                         uint32_t *plan)
@@ -873,8 +921,8 @@ __global__ __launch_bounds__(1024) void k_assign_groups(const uint8_t *__restric
         if (q < nq) {
             if (d) {
                 const uint32_t g = rank / G, b = rank - g * G;
-                qgroup[q] = g * 64 + b;
-                grp_q[g * 64 + b] = q;
+                qgroup[q] = g * kGroupSlots + b;
+                grp_q[g * kGroupSlots + b] = q;
             } else {
                 qgroup[q] = kNoGroup;
             }
@@ -924,7 +972,7 @@ __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict_
     const uint32_t q = blockIdx.y;
     const uint32_t qg = qgroup[q];
     if (qg == kNoGroup) return;
-    const uint32_t g = qg >> 6, b = qg & 63;
+    const uint32_t g = qg / kGroupSlots, b = qg % kGroupSlots;
     M *mask = reinterpret_cast<M *>(masks) + (uint64_t)g * mask_stride;
     uint32_t *any = anybms + (uint64_t)g * any_stride;
     const uint64_t a = qoff[q], e = qoff[q + 1];
@@ -933,10 +981,10 @@ __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict_
         if (k >= nkeys) continue;
         if (clear) {
             any[k >> 5] = 0;
-            mask[k] = 0;
+            mask[k] = M{};
         } else {
             atomicOr(&any[k >> 5], 1u << (k & 31));
-            atomicOr(&mask[k], (M)1 << b);
+            set_mask_bit(&mask[k], b);
         }
     }
 }
@@ -950,9 +998,12 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
     if (mode == kModeMask32)
         hipLaunchKernelGGL(k_mask_groups<kModeMask32>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks, mask_stride,
                            anybms, any_stride, nkeys, clear);
-    else
+    else if (mode == kModeMask64)
         hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks, mask_stride,
                            anybms, any_stride, nkeys, clear);
+    else
+        hipLaunchKernelGGL(k_mask_groups<kModeMask128>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks,
+                           mask_stride, anybms, any_stride, nkeys, clear);
 }
 
 __global__ void k_set_u32(uint32_t *p, uint32_t v)
@@ -982,7 +1033,7 @@ __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
         if (item < nd) {
             const uint32_t g = (uint32_t)(item / ix.ntiles), tile = (uint32_t)(item - (uint64_t)g * ix.ntiles);
             dense_tile<MODE, NT>(ix, tile, reinterpret_cast<const M *>(a.masks) + (uint64_t)g * a.mask_stride,
-                                 a.anybms + (uint64_t)g * a.any_stride, a.grp_n[g], a.grp_q + (uint64_t)g * 64,
+                                 a.anybms + (uint64_t)g * a.any_stride, a.grp_n[g], a.grp_q + (uint64_t)g * kGroupSlots,
                                  a.out, a.out_stride, a.out_words, a.use_col, lds);
         } else if (item < nd + ns) {
             const uint64_t c0 = (item - nd) * a.chunk, c1 = (c0 + a.chunk < W) ? c0 + a.chunk : W;
@@ -1017,8 +1068,10 @@ hipError_t fused_grid(int mode, uint32_t threads, const DevIndex &ix, uint32_t G
     const uint32_t lds = fused_lds_bytes(mode, G, ix.tile_shift, threads, use_col);
     if (mode == kModeMask32)
         e = threads == 256 ? occupancy_nt<kModeMask32, 256>(lds, &per) : occupancy_nt<kModeMask32, 512>(lds, &per);
-    else
+    else if (mode == kModeMask64)
         e = threads == 256 ? occupancy_nt<kModeMask64, 256>(lds, &per) : occupancy_nt<kModeMask64, 512>(lds, &per);
+    else
+        e = threads == 256 ? occupancy_nt<kModeMask128, 256>(lds, &per) : occupancy_nt<kModeMask128, 512>(lds, &per);
     if (e != hipSuccess) return e;
     *grid = (uint32_t)(cus * (per < 1 ? 1 : per));
     return hipSuccess;
@@ -1034,11 +1087,16 @@ hipError_t launch_union_fused(int mode, uint32_t threads, const DevIndex &ix, co
             hipLaunchKernelGGL((k_union_fused<kModeMask32, 256>), dim3(grid), dim3(256), lds, s, ix, a);
         else
             hipLaunchKernelGGL((k_union_fused<kModeMask32, 512>), dim3(grid), dim3(512), lds, s, ix, a);
-    } else {
+    } else if (mode == kModeMask64) {
         if (threads == 256)
             hipLaunchKernelGGL((k_union_fused<kModeMask64, 256>), dim3(grid), dim3(256), lds, s, ix, a);
         else
             hipLaunchKernelGGL((k_union_fused<kModeMask64, 512>), dim3(grid), dim3(512), lds, s, ix, a);
+    } else {
+        if (threads == 256)
+            hipLaunchKernelGGL((k_union_fused<kModeMask128, 256>), dim3(grid), dim3(256), lds, s, ix, a);
+        else
+            hipLaunchKernelGGL((k_union_fused<kModeMask128, 512>), dim3(grid), dim3(512), lds, s, ix, a);
     }
     return hipGetLastError();
 }

@@ -144,6 +144,9 @@ def main():
                     help="rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload (roofline traffic)")
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--tile-shift", type=int, default=0, help="value tile of the built index (0: automatic)")
+    ap.add_argument("--tile-shifts", default="10,12",
+                    help="value-tile sizes (log2) the autotuner also tries, one device-built index each")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -163,40 +166,49 @@ def main():
     pk, pv = configs.make_filter(spec)
     log(f"filter {spec.name}: {pk.size} pairs in {time.time() - t:.1f}s")
     t = time.time()
-    ix = ab.Index(pk, pv, spec.nkeys, spec.nvalues)
+    ix = ab.Index(pk, pv, spec.nkeys, spec.nvalues, args.tile_shift)
     info = ix.info
     log(f"index: R={info['nrec_total']} list_bytes={info['list_bytes']/1e6:.1f}MB key_stream="
         f"{info['key_stream_bytes']/1e6:.1f}MB tiles={info['ntiles']} device={info['device_bytes']/1e6:.0f}MB "
         f"in {time.time() - t:.1f}s")
     qk, qo = configs.make_queries(spec, configs.C2_Q, args.queries)
     nq = qo.size - 1
-    ctx = ab.Context(ix)
     path = {"auto": ab.AB_PATH_AUTO, "threshold": ab.AB_PATH_THRESHOLD, "dense": ab.AB_PATH_DENSE,
             "sparse": ab.AB_PATH_SPARSE, "learned": ab.AB_PATH_LEARNED}[args.path]
     dispatch = "threshold"
     model_meta = None
-    if os.path.exists(args.model):
-        d = ctx.load_model(args.model)
-        model_meta = {k: d[k].get("meta") for k in ("single", "batch") if k in d}
-        # thresholds calibrated on the same timings as the model
-        if "single" in d:
-            ctx.set_threshold(d["single"]["meta"]["tau"])
-        if "batch" in d:
-            ctx.set_batch_threshold(d["batch"]["meta"]["tau"])
-        if path in (ab.AB_PATH_AUTO, ab.AB_PATH_LEARNED):
-            dispatch = "learned"
 
-    if args.tau is not None:
-        ctx.set_threshold(args.tau)
-    if args.tau_batch is not None:
-        ctx.set_batch_threshold(args.tau_batch)
-    if args.dense_threads:
-        ctx.set_dense_threads(args.dense_threads)
-    if args.group:
-        ctx.set_dense_group(args.group)
-    if args.no_columns:
-        ctx.set_dense_columns(False)
-    ctx.set_batch_mode(args.batch_mode)
+    def setup_ctx(index):
+        """A context with the cost model, calibrated thresholds and the requested knobs."""
+        nonlocal dispatch, model_meta
+        c = ab.Context(index)
+        if os.path.exists(args.model):
+            d = c.load_model(args.model)
+            model_meta = {k: d[k].get("meta") for k in ("single", "batch") if k in d}
+            # thresholds calibrated on the same timings as the model
+            if "single" in d:
+                c.set_threshold(d["single"]["meta"]["tau"])
+            if "batch" in d:
+                c.set_batch_threshold(d["batch"]["meta"]["tau"])
+            if path in (ab.AB_PATH_AUTO, ab.AB_PATH_LEARNED):
+                dispatch = "learned"
+        if args.tau is not None:
+            c.set_threshold(args.tau)
+        if args.tau_batch is not None:
+            c.set_batch_threshold(args.tau_batch)
+        if args.dense_threads:
+            c.set_dense_threads(args.dense_threads)
+        if args.group:
+            c.set_dense_group(args.group)
+        if args.no_columns:
+            c.set_dense_columns(False)
+        c.set_batch_mode(args.batch_mode)
+        return c
+
+    ctx = setup_ctx(ix)
+    # the per-query baseline runs on this first index (automatic value tile),
+    # whatever the autotuner picks for the batches
+    base_ctx = ctx
     if args.calib_copy:
         n = args.calib_copy * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.int32, device="cuda")
@@ -216,12 +228,38 @@ def main():
     tuned = None
     explicit = args.dense_threads or args.group or args.no_columns
     if not args.no_autotune and not explicit:
+        # launch parameters on the built index, then the same for other value
+        # tiles (each index built on the device from the same pairs)
         t = time.time()
-        res = ctx.autotune(d_keys, qo, path, repeats=2)
+        res = base_ctx.autotune(d_keys, qo, path, repeats=2)
         dflt = next((r["ms"] for r in res if r["group"] == 64 and r["dense_threads"] == 512
                      and r["sparse_chunk"] == 4096 and r["dense_columns"]), None)
-        tuned = dict(res[0], default_ms=dflt, candidates=len(res))
-        log(f"autotune: {res[0]} of {len(res)} candidates (default {dflt}) in {time.time() - t:.1f}s")
+        tuned = dict(res[0], default_ms=dflt, candidates=len(res), tile_shift=info["tile_shift"])
+        by_tile = {info["tile_shift"]: res[0]["ms"]}
+        extra = [int(x) for x in args.tile_shifts.split(",") if x and int(x) != info["tile_shift"]]
+        if extra:
+            d_pk = torch.from_numpy(pk.view(np.int32)).cuda()
+            d_pv = torch.from_numpy(pv.view(np.int32)).cuda()
+            for ts in extra:
+                ix2 = ab.Index.build_device(d_pk, d_pv, spec.nkeys, spec.nvalues, ts)
+                ctx2 = setup_ctx(ix2)
+                r2 = ctx2.autotune(d_keys, qo, path, repeats=2)
+                by_tile[ts] = r2[0]["ms"]
+                if r2[0]["ms"] < tuned["ms"]:
+                    if ctx is not base_ctx:
+                        ctx.close()
+                        ix.close()
+                    ix, ctx = ix2, ctx2
+                    tuned.update(r2[0], tile_shift=ts, candidates=tuned["candidates"] + len(r2))
+                else:
+                    tuned["candidates"] += len(r2)
+                    ctx2.close()
+                    ix2.close()
+            del d_pk, d_pv
+            torch.cuda.empty_cache()
+            info = ix.info
+        tuned["ms_by_tile_shift"] = by_tile
+        log(f"autotune: {tuned} in {time.time() - t:.1f}s")
 
     def run(p, steps, warmup):
         """warmup + timed steps of the batch on dispatch p (no per-kernel events)."""
@@ -323,15 +361,15 @@ def main():
     d_c1 = torch.empty(1, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
     for i in range(min(8, nb)):
-        ctx.query_device(qk[qo[i]:qo[i + 1]], ab.AB_PATH_THRESHOLD, d_bm1, d_ids1, d_c1)
-        ctx.sync()
+        base_ctx.query_device(qk[qo[i]:qo[i + 1]], ab.AB_PATH_THRESHOLD, d_bm1, d_ids1, d_c1)
+        base_ctx.sync()
     lat1 = []
     mism = 0
     t1 = time.perf_counter()
     for i in range(nb):
         s0 = time.perf_counter()
-        ctx.query_device(qk[qo[i]:qo[i + 1]], ab.AB_PATH_THRESHOLD, d_bm1, d_ids1, d_c1)
-        ctx.sync()
+        base_ctx.query_device(qk[qo[i]:qo[i + 1]], ab.AB_PATH_THRESHOLD, d_bm1, d_ids1, d_c1)
+        base_ctx.sync()
         lat1.append((time.perf_counter() - s0) * 1e3)
         if i < 16:
             mism += int(int(d_c1.cpu().numpy().view(np.uint32)[0]) != counts[i])
@@ -368,7 +406,8 @@ def main():
                        "per_query_threshold_p50": float(np.percentile(lat1, 50)),
                        "per_query_threshold_p99": float(np.percentile(lat1, 99))},
         "per_query_threshold_qps": round(base_qps, 1),
-        "per_query_threshold_tau": ctx.threshold,
+        "per_query_threshold_tau": base_ctx.threshold,
+        "per_query_threshold_tile_shift": base_ctx.index.info["tile_shift"],
         "batched_threshold": thr_cmp,
         "batch_modes_qps": mode_cmp,
         "autotune": tuned,

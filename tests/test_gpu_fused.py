@@ -98,6 +98,7 @@ def test_graph_recapture_on_changes(c1):
         ("tau", lambda: ctx.set_batch_threshold(0.002)),
         ("group 7", lambda: ctx.set_dense_group(7)),
         ("group 33", lambda: ctx.set_dense_group(33)),
+        ("group 128", lambda: ctx.set_dense_group(128)),
         ("threads 256", lambda: ctx.set_dense_threads(256)),
         ("no columns", lambda: ctx.set_dense_columns(False)),
         ("chunk 256", lambda: ctx.set_sparse_chunk(256)),
@@ -159,10 +160,21 @@ def test_fused_edge_filter(ts):
     for mode in (ab.Context.BATCH_FUSED, ab.Context.BATCH_GRAPH):
         ctx = ab.Context(ix)
         ctx.set_batch_mode(mode)
-        for group in (1, 2, 32, 64):
+        for group in (1, 2, 32, 64, 100, 128):
             ctx.set_dense_group(group)
             for path in (ab.AB_PATH_DENSE, ab.AB_PATH_SPARSE):
                 print(f"edge ts={ts} mode={mode} group={group} path={path}", flush=True)
                 res = ctx.query_batch(qk, qo, path)
                 _verify(csr, qk, qo, res, f"ts={ts} mode={mode} group={group} path={path}")
         ctx.close()
+
+
+def test_host_dispatch_caps_group_width(c1):
+    ix, csr = c1
+    ctx = ab.Context(ix)
+    ctx.set_batch_mode(ab.Context.BATCH_HOST)
+    ctx.set_dense_group(128)
+    qk, qo = _mixed(0xADAB0205, 150, configs.C1.nkeys)
+    res = ctx.query_batch(qk, qo, ab.AB_PATH_DENSE)
+    _verify(csr, qk, qo, res, "host dispatch, group 128")
+    ctx.close()
