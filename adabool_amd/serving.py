@@ -44,6 +44,8 @@ def main(argv=None):
     ap.add_argument("--deliver", choices=["device", "host"], default="device")
     ap.add_argument("--baseline-queries", type=int, default=2000)
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
+    ap.add_argument("--tile-shifts", default="10", help="value tiles the autotuner also tries (device builds)")
+    ap.add_argument("--no-autotune", action="store_true")
     a = ap.parse_args(argv)
 
     import torch
@@ -69,18 +71,49 @@ def main(argv=None):
     same = ixd.info["key_stream_bytes"] == info["key_stream_bytes"] and ixd.info["nrec_total"] == info["nrec_total"]
     log(f"index (device build): {t_dev:.2f}s, same shape: {same}")
     ixd.close()
-    del dk, dv, pk, pv
-    torch.cuda.empty_cache()
     qk, qo = configs.make_queries(spec, configs.C3_Q, a.queries)
     nq = qo.size - 1
     sizes = np.diff(qo)
     log(f"{nq} queries: |q| median {np.median(sizes):.0f}, mean {sizes.mean():.0f}, max {sizes.max()}")
 
+    # ---- autotune on a sample batch: launch parameters, then other value tiles
+    tuned = None
+    serve_ix = ix
+    if not a.no_autotune:
+        ns = min(nq, 256)
+        sk = torch.from_numpy(np.ascontiguousarray(qk[: int(qo[ns])]).view(np.int32)).cuda()
+        so = qo[: ns + 1]
+
+        def tune_on(index):
+            c = Context(index)
+            if os.path.exists(a.model):
+                c.load_model(a.model)
+            r = c.autotune(sk, so, AB_PATH_AUTO, repeats=2)[0]
+            c.close()
+            return r
+
+        best = dict(tune_on(ix), tile_shift=info["tile_shift"])
+        for ts in [int(x) for x in a.tile_shifts.split(",") if x and int(x) != info["tile_shift"]]:
+            cand = Index.build_device(dk, dv, spec.nkeys, spec.nvalues, ts)
+            r = tune_on(cand)
+            log(f"autotune tile 2^{ts}: {r['ms']:.3f} ms vs {best['ms']:.3f} ms")
+            if r["ms"] < best["ms"]:
+                if serve_ix is not ix:
+                    serve_ix.close()
+                serve_ix, best = cand, dict(r, tile_shift=ts)
+            else:
+                cand.close()
+        tuned = best
+        log(f"autotune: {tuned}")
+    del dk, dv, pk, pv
+    torch.cuda.empty_cache()
+
     out = {"workload": f"{spec.name}: {spec.nvalues} values x {spec.nkeys} keys, {info['nnz']} pairs, Zipf "
                        f"{spec.zipf_s}; {nq} C3-mix queries (power law 1.5 on [1, 50000], Zipf-weighted keys), "
                        f"Poisson arrivals",
            "deliver": a.deliver, "max_batch": a.max_batch, "inflight": a.inflight, "max_wait_us": a.max_wait_us,
-           "batch_mode": a.batch_mode, "index_build_s": {"host": round(t_host, 2), "device": round(t_dev, 3)}}
+           "batch_mode": a.batch_mode, "index_build_s": {"host": round(t_host, 2), "device": round(t_dev, 3)},
+           "autotune": tuned}
 
     # ---- per-query synchronous threshold dispatch (the x5 yardstick) -------
     ctx = Context(ix)
@@ -113,12 +146,15 @@ def main(argv=None):
     torch.cuda.empty_cache()
 
     # ---- serving loop --------------------------------------------------------
-    srv = Server(ix, max_batch=a.max_batch, max_wait_us=a.max_wait_us, inflight=a.inflight,
+    srv = Server(serve_ix, max_batch=a.max_batch, max_wait_us=a.max_wait_us, inflight=a.inflight,
                  arena_bytes=int(a.arena_gb * (1 << 30)), path=AB_PATH_AUTO,
                  deliver=Server.DELIVER_HOST if a.deliver == "host" else Server.DELIVER_DEVICE,
                  batch_mode=a.batch_mode)
     if os.path.exists(a.model):
         srv.load_model(a.model)
+    if tuned:
+        for c in srv.contexts:
+            c.apply_tuning(tuned)
     # warm-up (allocations, code objects)
     srv.replay(qk[: int(qo[min(nq, 2048)])], qo[: min(nq, 2048) + 1], 0.0, 7)
     srv.stats(reset=True)
