@@ -35,7 +35,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--scale", type=float, default=1.0, help="C5 size factor")
     ap.add_argument("--queries", type=int, default=20000)
-    ap.add_argument("--rates", default="50000,100000,200000,500000")
+    ap.add_argument("--rates", default="10000,20000,50000,100000,200000,500000")
     ap.add_argument("--max-batch", type=int, default=1024)
     ap.add_argument("--max-wait-us", type=int, default=200)
     ap.add_argument("--inflight", type=int, default=2)
