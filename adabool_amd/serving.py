@@ -40,7 +40,7 @@ def main(argv=None):
     ap.add_argument("--max-wait-us", type=int, default=200)
     ap.add_argument("--inflight", type=int, default=2)
     ap.add_argument("--batch-mode", type=int, default=1)
-    ap.add_argument("--arena-gb", type=float, default=32.0)
+    ap.add_argument("--arena-gb", type=float, default=128.0)
     ap.add_argument("--deliver", choices=["device", "host"], default="device")
     ap.add_argument("--baseline-queries", type=int, default=2000)
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
