@@ -38,13 +38,13 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 __all__ = [
-    "AB_PATH_AUTO", "AB_PATH_DENSE", "AB_PATH_SPARSE", "AB_PATH_THRESHOLD", "AB_PATH_LEARNED",
+    "AB_PATH_AUTO", "AB_PATH_DENSE", "AB_PATH_SPARSE", "AB_PATH_THRESHOLD", "AB_PATH_LEARNED", "AB_PATH_IDS",
     "NTYPES", "PAYLOAD_BYTES", "NFEATURES", "AdaBoolError", "lib_path", "load_library",
     "Index", "Context", "IvfList", "gen_pairs", "gen_queries", "read_coo", "read_query",
     "stats", "key_block_meta",
 ]
 
-AB_PATH_AUTO, AB_PATH_DENSE, AB_PATH_SPARSE, AB_PATH_THRESHOLD, AB_PATH_LEARNED = range(5)
+AB_PATH_AUTO, AB_PATH_DENSE, AB_PATH_SPARSE, AB_PATH_THRESHOLD, AB_PATH_LEARNED, AB_PATH_IDS = range(6)
 MODEL_SINGLE, MODEL_BATCH = 0, 1
 NTYPES = 6
 PAYLOAD_BYTES = (0, 4, 8, 16, 32, 32)
@@ -119,6 +119,9 @@ _SIGS = {
     "ab_query_batch": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, _VP, _VP, C.c_uint64, _VP, _VP]),
     "ab_compact_ids": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint64, _VP]),
     "ab_query_batch_packed": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, _VP, _VP, _VP, _VP, _VP]),
+    "ab_query_batch_ids": (C.c_int, [_VP, _VP, _VP, C.c_uint32, _VP, _VP, _VP]),
+    "ab_query_id_bound": (C.c_int, [_VP, _VP, C.c_uint32, _VP]),
+    "ab_ctx_set_id_budget": (C.c_int, [_VP, C.c_uint64]),
     "ab_ctx_autotune": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, C.c_uint32, _VP, C.c_uint32, _VP]),
     "ab_server_config_default": (C.c_int, [_VP]),
     "ab_server_create": (C.c_int, [_VP, _VP, _VP]),
@@ -394,6 +397,14 @@ class Index:
     def stride_words(self) -> int:
         return self.stride_bytes // 4
 
+    def id_bound(self, keys) -> int:
+        """Upper bound of a query's id count (its keys' list lengths, capped
+        at nvalues)."""
+        k = _u32(keys)
+        b = C.c_uint64()
+        _check(_lib.ab_query_id_bound(self._h, _ptr(k), k.size, C.byref(b)))
+        return b.value
+
     def export_type(self, t: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
         """(headers, payload bytes, key_segments prefix) of record type t."""
         inf = self.info
@@ -491,7 +502,7 @@ class Context:
         _check(_lib.ab_ctx_sync(self._h))
 
     KERNEL_CLASSES = ("decide", "sparse_zero", "sparse_plan", "sparse_union", "dense_mask", "dense_tiles",
-                      "popcount", "write_ids", "fused_union")
+                      "popcount", "write_ids", "fused_union", "ids")
 
     def set_timing(self, on: bool = True):
         _check(_lib.ab_ctx_set_timing(self._h, 1 if on else 0))
@@ -565,6 +576,40 @@ class Context:
         _check(_lib.ab_query_batch_packed(self._h, _ptr(d_keys), _ptr(off), nq, path, _ptr(d_bitmaps), _ptr(d_ids),
                                           _ptr(ido), _ptr(d_counts), _ptr(paths)))
         return paths[:nq]
+
+    def set_id_budget(self, values: int):
+        _check(_lib.ab_ctx_set_id_budget(self._h, int(values)))
+
+    def query_batch_ids_device(self, d_keys, qoff, d_ids, id_off, d_counts=None):
+        """Id-only batch (no bitmaps): query i's ascending ids at
+        d_ids + id_off[i] (u32 units), its count at d_counts[i]."""
+        off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
+        ido = np.ascontiguousarray(np.asarray(id_off, dtype=np.uint64))
+        _check(_lib.ab_query_batch_ids(self._h, _ptr(d_keys), _ptr(off), off.size - 1, _ptr(d_ids), _ptr(ido),
+                                       _ptr(d_counts)))
+
+    def query_batch_ids(self, keys, qoff):
+        """Id-only batch of host queries: (list of id arrays, counts).  Each
+        query gets room for its id upper bound (Index.id_bound)."""
+        torch = _torch()
+        k = _u32(keys)
+        off = np.asarray(qoff, dtype=np.uint64)
+        nq = off.size - 1
+        ub = np.array([self.index.id_bound(k[off[q]:off[q + 1]]) for q in range(nq)], np.uint64)
+        room = (ub + 3) // 4 * 4
+        ido = np.zeros(max(nq, 1), np.uint64)
+        if nq:
+            ido[1:nq] = np.cumsum(room)[:-1]
+        total = int(room.sum()) if nq else 0
+        dk = torch.from_numpy(k.view(np.int32)).cuda() if k.size else torch.zeros(1, dtype=torch.int32, device="cuda")
+        ids = torch.empty(max(total, 4), dtype=torch.int32, device="cuda")
+        cnt = torch.empty(max(nq, 1), dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        self.query_batch_ids_device(dk, off, ids, ido[:nq], cnt)
+        self.sync()
+        counts = cnt[:nq].cpu().numpy().view(np.uint32).astype(np.int64)
+        ih = ids.cpu().numpy().view(np.uint32)
+        return [ih[int(ido[q]): int(ido[q]) + int(counts[q])].copy() for q in range(nq)], counts
 
     def query_batch(self, keys, qoff, path: int = AB_PATH_THRESHOLD, want_ids: bool = True):
         """Batch of queries (host arrays in, host arrays out):
@@ -670,7 +715,8 @@ class _TuneConfig(C.Structure):
 
 class _ServerConfig(C.Structure):
     _fields_ = [("max_batch", C.c_uint32), ("max_wait_us", C.c_uint32), ("inflight", C.c_uint32),
-                ("arena_bytes", C.c_uint64), ("path", C.c_int), ("deliver", C.c_int), ("batch_mode", C.c_int)]
+                ("arena_bytes", C.c_uint64), ("path", C.c_int), ("deliver", C.c_int), ("batch_mode", C.c_int),
+                ("ids_only", C.c_int)]
 
 
 class _Result(C.Structure):

@@ -90,6 +90,11 @@ struct ab_ctx {
     int batch_mode = 2;
     uint64_t cfg_gen = 1;  // bumped by every setting that changes launch parameters
     ab::DevBuf qgroup, grp_q, grp_n, plan, counter, gmask, ganybm;
+    // id-only batches (ab_query_batch_ids): values expanded per part of at
+    // most id_budget values, chunk counts and bases, sort output, run starts,
+    // rocPRIM temporary storage
+    uint64_t id_budget = 1ull << 27;
+    ab::DevBuf id_vals, id_sorted, id_ccnt, id_cbase, id_start, id_tmp;
     uint32_t gmask_groups = 0;  // groups the zeroed mask buffers hold
     int gmask_mode = -1;
     struct GraphCache {
@@ -135,6 +140,9 @@ ab_status run_batch(ab_ctx *c, const BatchArgs &a);
 // replayed from a cached hipGraph (fused.hip).
 ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a);
 void release_graph(ab_ctx *c);
+// Upper bound of a query's id count: the sum of its in-range keys' list
+// lengths, capped at nvalues (host seg8 copy).
+uint64_t id_upper_bound(const ab_index *ix, const uint32_t *keys, uint64_t n);
 // Largest dense group width that fits the LDS budget and whether its
 // per-value column masks fit as well.
 uint32_t max_group(const ab_ctx *c);

@@ -1,0 +1,450 @@
+// ids.hip -- id-only batches: every query's ascending ids formed without a
+// result bitmap.  Memory and traffic follow the values that the query's lists
+// hold, not nvalues, so this serves indices whose per-query bitmaps would not
+// fit (nvalues up to 2^32 - 1) and queries whose results are a small part of
+// nvalues.
+//
+//   1. plan, one work-group per query: each key's record count from seg8
+//      (the Prepare(keys) extents, src/AdaBool/inverted_list.h:491-505), the
+//      query's record and value totals; the totals go back to the host, which
+//      splits the batch into parts that fit the expansion budget;
+//   2. per part: an exclusive scan numbers the records; work-groups take
+//      chunks of kIdsChunk records, count their values (record_count), and a
+//      scan over chunks places each chunk's values;
+//   3. the same chunks again: a work-group scan of the record counts places
+//      every record, whose values go out as (query << 32 | value);
+//   4. radix sort and unique of the packed values (rocPRIM);
+//   5. the first index of every query's run, then each value to the caller's
+//      offset for its query, and the run length as its count.
+// The result equals the bitmap paths' ids for every query: the set union of
+// its in-range keys' lists, ascending (test/adsfilter_q_all.cu:108-119,
+// 196-208).
+#include <cstring>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/rocprim.hpp>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "engine.h"
+#include "layout.h"
+
+using namespace ab;
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));            \
+            return e_ == hipErrorOutOfMemory ? AB_OOM : AB_DEVICE_ERROR;              \
+        }                                                                             \
+    } while (0)
+#define AB_TRY(expr)                \
+    do {                            \
+        ab_status s_ = (expr);      \
+        if (s_ != AB_OK) return s_; \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t kIdsNT = 256;      // threads per work-group
+constexpr uint32_t kIdsChunk = 4096;  // records per chunk
+constexpr uint32_t kIdsWin = 1024;    // key offsets per LDS window
+constexpr uint32_t kIdsGrid = 8192;   // work-groups of the chunk kernels (grid-stride)
+
+// Exclusive work-group scan of one u32 per thread; *total = the sum.
+__device__ __forceinline__ uint32_t block_excl_u32(uint32_t x, uint32_t *sw, uint32_t *total)
+{
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_incl_scan(x);
+    if (lane == 63) sw[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kIdsNT / 64; ++w) {
+        before += w < wave ? sw[w] : 0u;
+        all += sw[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - x;
+}
+
+// Per query q: rcnt[j] = records of key j (0 when out of range), kq[j] = q,
+// tot[2q] = sum of rcnt over the query, tot[2q+1] = its values (list lengths
+// with repeats).
+__global__ __launch_bounds__(kIdsNT) void k_ids_plan(DevIndex ix, const uint32_t *__restrict__ keys,
+                                                     const uint64_t *__restrict__ qoff, uint32_t *__restrict__ kq,
+                                                     uint64_t *__restrict__ rcnt, uint64_t *__restrict__ tot)
+{
+    __shared__ unsigned long long sr[kIdsNT / 64], sv[kIdsNT / 64];
+    const uint32_t q = blockIdx.x, lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t a = qoff[q], b = qoff[q + 1];
+    unsigned long long r = 0, v = 0;
+    for (uint64_t j = a + threadIdx.x; j < b; j += kIdsNT) {
+        const uint32_t k = keys[j];
+        uint64_t c = 0;
+        if (k < ix.nkeys) {
+            const uint32_t *lo = ix.seg8 + (uint64_t)k * kSegStride, *hi = lo + kSegStride;
+            c = hi[6] - lo[6];
+            v += hi[7] - lo[7];
+        }
+        rcnt[j] = c;
+        kq[j] = q;
+        r += c;
+    }
+    r = wave_sum64(r);
+    v = wave_sum64(v);
+    if (lane == 0) {
+        sr[wave] = r;
+        sv[wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long R = 0, V = 0;
+        for (uint32_t w = 0; w < kIdsNT / 64; ++w) {
+            R += sr[w];
+            V += sv[w];
+        }
+        tot[2ull * q] = R;
+        tot[2ull * q + 1] = V;
+    }
+}
+
+// One record, located by key j's seg8 rows and its rank r among the key's
+// records (types in order ARRAY1..BITSET).
+struct Rec {
+    int t;
+    uint32_t h;
+    uint32_t w[8];
+};
+
+template <int T>
+__device__ __forceinline__ void load_rec(const DevIndex &ix, uint32_t idx, Rec &rc)
+{
+    rc.t = T;
+    rc.h = ix.hdr[T][idx];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rc.w[i] = 0;
+    load_payload<T>(ix.pay[T], idx, rc.w);
+}
+
+__device__ __forceinline__ void locate(const DevIndex &ix, uint32_t k, uint32_t r, Rec &rc)
+{
+    const uint4 *rows = reinterpret_cast<const uint4 *>(ix.seg8 + (uint64_t)k * kSegStride);
+    const uint4 a0 = rows[0], a1 = rows[1], b0 = rows[2], b1 = rows[3];
+    const uint32_t l0 = b0.x - a0.x, l1 = b0.y - a0.y, l2 = b0.z - a0.z, l3 = b0.w - a0.w, l4 = b1.x - a1.x;
+    if (r < l0) return load_rec<AB_ARRAY1>(ix, a0.x + r, rc);
+    r -= l0;
+    if (r < l1) return load_rec<AB_ARRAY4>(ix, a0.y + r, rc);
+    r -= l1;
+    if (r < l2) return load_rec<AB_ARRAY8>(ix, a0.z + r, rc);
+    r -= l2;
+    if (r < l3) return load_rec<AB_ARRAY16>(ix, a0.w + r, rc);
+    r -= l3;
+    if (r < l4) return load_rec<AB_ARRAY32>(ix, a1.x + r, rc);
+    r -= l4;
+    load_rec<AB_BITSET>(ix, a1.y + r, rc);
+}
+
+__device__ __forceinline__ uint32_t rec_count(const Rec &rc)
+{
+    switch (rc.t) {
+    case AB_ARRAY1: return record_count<AB_ARRAY1>(rc.w);
+    case AB_ARRAY4: return record_count<AB_ARRAY4>(rc.w);
+    case AB_ARRAY8: return record_count<AB_ARRAY8>(rc.w);
+    case AB_ARRAY16: return record_count<AB_ARRAY16>(rc.w);
+    case AB_ARRAY32: return record_count<AB_ARRAY32>(rc.w);
+    default: return record_count<AB_BITSET>(rc.w);
+    }
+}
+
+template <class F>
+__device__ __forceinline__ void rec_values(const Rec &rc, F &&f)
+{
+    switch (rc.t) {
+    case AB_ARRAY1: record_values<AB_ARRAY1>(rc.h, rc.w, f); break;
+    case AB_ARRAY4: record_values<AB_ARRAY4>(rc.h, rc.w, f); break;
+    case AB_ARRAY8: record_values<AB_ARRAY8>(rc.h, rc.w, f); break;
+    case AB_ARRAY16: record_values<AB_ARRAY16>(rc.h, rc.w, f); break;
+    case AB_ARRAY32: record_values<AB_ARRAY32>(rc.h, rc.w, f); break;
+    default: record_values<AB_BITSET>(rc.h, rc.w, f); break;
+    }
+}
+
+// Records [c0, c1) of a part, in rounds of kIdsNT (one record per thread).
+// koff: the part's record prefix over its K keys.  EMIT = false: returns the
+// chunk's value count; EMIT = true: writes the chunk's values from out[base]
+// on, in record order, as (query - q0) << 32 | value (never at or past limit).
+template <bool EMIT>
+__device__ uint64_t ids_chunk(const DevIndex &ix, const uint32_t *__restrict__ keys,
+                              const uint32_t *__restrict__ kq, const uint64_t *__restrict__ koff, uint64_t K,
+                              uint64_t c0, uint64_t c1, uint32_t q0, unsigned long long *__restrict__ out,
+                              uint64_t base, uint64_t limit, unsigned long long *skoff, unsigned long long *s_lo,
+                              unsigned long long *s_hi, uint32_t *sw)
+{
+    uint64_t jw = wg_find<kIdsNT>(koff, K, c0, s_lo, s_hi);
+    uint64_t run = 0;
+    for (;;) {
+        const uint64_t jn = (jw + kIdsWin < K) ? jw + kIdsWin : K;
+        const uint32_t nwin = (uint32_t)(jn - jw);
+        for (uint32_t i = threadIdx.x; i <= nwin; i += kIdsNT) skoff[i] = koff[jw + i];
+        __syncthreads();
+        const uint64_t wlo = skoff[0] > c0 ? skoff[0] : c0;
+        const uint64_t whi = skoff[nwin] < c1 ? skoff[nwin] : c1;
+        for (uint64_t it0 = wlo; it0 < whi; it0 += kIdsNT) {
+            const uint64_t it = it0 + threadIdx.x;
+            Rec rc;
+            uint32_t cnt = 0, q = 0;
+            if (it < whi) {
+                uint32_t lo = 0, hi = nwin;  // skoff[lo] <= it < skoff[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (skoff[mid] <= it) lo = mid; else hi = mid;
+                }
+                const uint64_t j = jw + lo;
+                q = kq[j] - q0;
+                locate(ix, keys[j], (uint32_t)(it - skoff[lo]), rc);
+                cnt = rec_count(rc);
+            }
+            uint32_t tot;
+            const uint32_t ex = block_excl_u32(cnt, sw, &tot);
+            if (EMIT && it < whi) {
+                uint64_t pos = base + run + ex;
+                const unsigned long long qk = (unsigned long long)q << 32;
+                rec_values(rc, [&](uint32_t v) {
+                    if (pos < limit) out[pos] = qk | v;
+                    ++pos;
+                });
+            }
+            run += tot;
+        }
+        const bool done = (skoff[nwin] >= c1) || jn >= K;
+        __syncthreads();
+        if (done) break;
+        jw = jn;
+    }
+    return run;
+}
+
+__global__ __launch_bounds__(kIdsNT) void k_ids_count(DevIndex ix, const uint32_t *__restrict__ keys,
+                                                      const uint32_t *__restrict__ kq,
+                                                      const uint64_t *__restrict__ koff, uint64_t K, uint64_t W,
+                                                      uint64_t *__restrict__ ccnt)
+{
+    __shared__ unsigned long long skoff[kIdsWin + 1];
+    __shared__ unsigned long long s_lo, s_hi;
+    __shared__ uint32_t sw[kIdsNT / 64];
+    const uint64_t nch = (W + kIdsChunk - 1) / kIdsChunk;
+    for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint64_t c0 = c * kIdsChunk, c1 = (c0 + kIdsChunk < W) ? c0 + kIdsChunk : W;
+        const uint64_t n = ids_chunk<false>(ix, keys, kq, koff, K, c0, c1, 0, nullptr, 0, 0, skoff, &s_lo, &s_hi, sw);
+        if (threadIdx.x == 0) ccnt[c] = n;
+    }
+}
+
+__global__ __launch_bounds__(kIdsNT) void k_ids_expand(DevIndex ix, const uint32_t *__restrict__ keys,
+                                                       const uint32_t *__restrict__ kq,
+                                                       const uint64_t *__restrict__ koff, uint64_t K, uint64_t W,
+                                                       const uint64_t *__restrict__ cbase, uint32_t q0,
+                                                       unsigned long long *__restrict__ out, uint64_t limit)
+{
+    __shared__ unsigned long long skoff[kIdsWin + 1];
+    __shared__ unsigned long long s_lo, s_hi;
+    __shared__ uint32_t sw[kIdsNT / 64];
+    const uint64_t nch = (W + kIdsChunk - 1) / kIdsChunk;
+    for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint64_t c0 = c * kIdsChunk, c1 = (c0 + kIdsChunk < W) ? c0 + kIdsChunk : W;
+        (void)ids_chunk<true>(ix, keys, kq, koff, K, c0, c1, q0, out, cbase[c], limit, skoff, &s_lo, &s_hi, sw);
+    }
+}
+
+__global__ void k_ids_zero(uint32_t *p, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// start[q] = index of query q's first value in the sorted unique array u.
+__global__ void k_ids_starts(const unsigned long long *__restrict__ u, const uint64_t *__restrict__ nu,
+                             uint64_t *__restrict__ start)
+{
+    const uint64_t n = *nu;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = (uint32_t)(u[i] >> 32);
+        if (i == 0 || (uint32_t)(u[i - 1] >> 32) != q) start[q] = i;
+    }
+}
+
+// ids[id_off[q0 + q] + i - start[q]] = value; counts[q0 + q] = run length.
+__global__ void k_ids_write(const unsigned long long *__restrict__ u, const uint64_t *__restrict__ nu,
+                            const uint64_t *__restrict__ start, const uint64_t *__restrict__ id_off, uint32_t q0,
+                            uint32_t *__restrict__ ids, uint32_t *__restrict__ counts)
+{
+    const uint64_t n = *nu;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const unsigned long long x = u[i];
+        const uint32_t q = (uint32_t)(x >> 32);
+        const uint64_t s = start[q];
+        ids[id_off[q0 + q] + (i - s)] = (uint32_t)x;
+        if (i + 1 == n || (uint32_t)(u[i + 1] >> 32) != q) counts[q0 + q] = (uint32_t)(i + 1 - s);
+    }
+}
+
+int bits_for(uint64_t x)
+{
+    int b = 0;
+    while (x) {
+        ++b;
+        x >>= 1;
+    }
+    return b;
+}
+
+}  // namespace
+
+uint64_t ab::id_upper_bound(const ab_index *ix, const uint32_t *keys, uint64_t n)
+{
+    uint64_t ub = 0;
+    const uint32_t *seg = ix->seg8.data();
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t k = keys[i];
+        if (k >= ix->nkeys) continue;
+        ub += seg[((uint64_t)k + 1) * kSegStride + 7] - seg[(uint64_t)k * kSegStride + 7];
+        if (ub >= ix->nvalues) return ix->nvalues;
+    }
+    return ub;
+}
+
+extern "C" ab_status ab_query_id_bound(const ab_index *ix, const uint32_t *h_keys, uint32_t n, uint64_t *bound)
+{
+    if (!ix || !bound || (n && !h_keys)) {
+        set_error("ab_query_id_bound: bad argument");
+        return AB_INVALID_ARG;
+    }
+    *bound = id_upper_bound(ix, h_keys, n);
+    return AB_OK;
+}
+
+extern "C" ab_status ab_ctx_set_id_budget(ab_ctx *c, uint64_t values)
+{
+    if (!c || values < kIdsChunk) {
+        set_error("ab_ctx_set_id_budget: budget below one chunk");
+        return AB_INVALID_ARG;
+    }
+    c->id_budget = values;
+    return AB_OK;
+}
+
+extern "C" ab_status ab_query_batch_ids(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq,
+                                        uint32_t *d_ids, const uint64_t *h_id_off, uint32_t *d_counts)
+{
+    if (!c || !h_qoff || !d_ids || !h_id_off || (nq && h_qoff[0] != 0) || c->ix->host_only) {
+        set_error("ab_query_batch_ids: bad argument");
+        return AB_INVALID_ARG;
+    }
+    if (nq == 0) return AB_OK;
+    for (uint32_t q = 0; q < nq; ++q)
+        if (h_qoff[q + 1] < h_qoff[q]) {
+            set_error("ab_query_batch_ids: offsets must not decrease");
+            return AB_INVALID_ARG;
+        }
+    const uint64_t K = h_qoff[nq];
+    if (K && !d_keys) {
+        set_error("ab_query_batch_ids: NULL keys");
+        return AB_INVALID_ARG;
+    }
+    ab_index *ix = c->ix;
+    const DevIndex &d = ix->dev;
+    hipStream_t s = c->stream;
+    TimedSpan span(c, AB_K_IDS);
+
+    // ---- plan: per-key record counts, per-query totals back to the host ------
+    AB_TRY(stage_wait(c));
+    const size_t qoff_bytes = ((size_t)nq + 1) * 8, idoff_bytes = (size_t)nq * 8;
+    AB_TRY(c->hstage.ensure(qoff_bytes + idoff_bytes + 64));
+    memcpy(c->hstage.p, h_qoff, qoff_bytes);
+    memcpy((uint8_t *)c->hstage.p + qoff_bytes, h_id_off, idoff_bytes);
+    AB_TRY(c->qoff.ensure(qoff_bytes));
+    AB_TRY(c->idoff.ensure(idoff_bytes));
+    HIP_TRY(hipMemcpyAsync(c->qoff.p, c->hstage.p, qoff_bytes, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->idoff.p, (uint8_t *)c->hstage.p + qoff_bytes, idoff_bytes, hipMemcpyHostToDevice, s));
+    AB_TRY(c->kq.ensure(K * 4 + 4));
+    AB_TRY(c->cnt.ensure(K * 8 + 8));
+    AB_TRY(c->M.ensure((size_t)nq * 16));
+    AB_TRY(c->hM.ensure((size_t)nq * 16));
+    hipLaunchKernelGGL(k_ids_plan, dim3(nq), dim3(kIdsNT), 0, s, d, d_keys, (const uint64_t *)c->qoff.p,
+                       (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, (uint64_t *)c->M.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(c->hM.p, c->M.p, (size_t)nq * 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<uint64_t> tot((const uint64_t *)c->hM.p, (const uint64_t *)c->hM.p + 2ull * nq);
+
+    uint32_t *counts = d_counts;
+    if (!counts) {
+        AB_TRY(c->ccnt.ensure((size_t)nq * 4 + 4));
+        counts = (uint32_t *)c->ccnt.p;
+    }
+    hipLaunchKernelGGL(k_ids_zero, dim3((nq + 255) / 256), dim3(256), 0, s, counts, nq);
+
+    // ---- parts of at most id_budget values (a larger query forms its own) ----
+    const uint64_t budget = c->id_budget;
+    uint32_t q0 = 0;
+    while (q0 < nq) {
+        uint64_t T = tot[2ull * q0 + 1], W = tot[2ull * q0];
+        uint32_t q1 = q0 + 1;
+        while (q1 < nq && T + tot[2ull * q1 + 1] <= budget) {
+            T += tot[2ull * q1 + 1];
+            W += tot[2ull * q1];
+            ++q1;
+        }
+        if (T == 0) {
+            q0 = q1;
+            continue;
+        }
+        const uint64_t k0 = h_qoff[q0], Kp = h_qoff[q1] - k0;
+        const uint64_t nch = (W + kIdsChunk - 1) / kIdsChunk;
+        // record numbering, chunk value counts, chunk bases
+        AB_TRY(c->koff.ensure((Kp + 1) * 8));
+        AB_TRY(c->parts.ensure((scan_parts_needed(std::max(Kp, nch)) + 2) * 8));
+        AB_TRY(c->id_cbase.ensure((nch + 1) * 8));
+        AB_TRY(c->id_ccnt.ensure(nch * 8 + 8));
+        AB_TRY(c->id_vals.ensure(T * 8 + 64));
+        AB_TRY(c->id_sorted.ensure(T * 8 + 64));
+        AB_TRY(c->id_start.ensure((size_t)(q1 - q0) * 8 + 8));
+        launch_scan_u64((const uint64_t *)c->cnt.p + k0, (uint64_t *)c->koff.p, Kp, (uint64_t *)c->parts.p, s);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(nch, kIdsGrid);
+        const uint32_t *keys_p = d_keys + k0, *kq_p = (const uint32_t *)c->kq.p + k0;
+        hipLaunchKernelGGL(k_ids_count, dim3(grid), dim3(kIdsNT), 0, s, d, keys_p, kq_p,
+                           (const uint64_t *)c->koff.p, Kp, W, (uint64_t *)c->id_ccnt.p);
+        HIP_TRY(hipGetLastError());
+        launch_scan_u64((const uint64_t *)c->id_ccnt.p, (uint64_t *)c->id_cbase.p, nch, (uint64_t *)c->parts.p, s);
+        hipLaunchKernelGGL(k_ids_expand, dim3(grid), dim3(kIdsNT), 0, s, d, keys_p, kq_p,
+                           (const uint64_t *)c->koff.p, Kp, W, (const uint64_t *)c->id_cbase.p, q0,
+                           (unsigned long long *)c->id_vals.p, T);
+        HIP_TRY(hipGetLastError());
+        // sort + unique (query-major, values ascending)
+        const int end_bit = 32 + bits_for(q1 - q0 - 1);
+        unsigned long long *vals = (unsigned long long *)c->id_vals.p, *srt = (unsigned long long *)c->id_sorted.p;
+        uint64_t *nu = (uint64_t *)c->id_ccnt.p + nch;  // one u64 after the chunk counts
+        size_t tb = 0, tb2 = 0;
+        HIP_TRY(rocprim::radix_sort_keys(nullptr, tb, vals, srt, (size_t)T, 0, end_bit, s));
+        HIP_TRY(rocprim::unique(nullptr, tb2, srt, vals, nu, (size_t)T, rocprim::equal_to<unsigned long long>(), s));
+        AB_TRY(c->id_tmp.ensure(std::max(tb, tb2) + 64));
+        tb = std::max(tb, tb2);
+        HIP_TRY(rocprim::radix_sort_keys(c->id_tmp.p, tb, vals, srt, (size_t)T, 0, end_bit, s));
+        HIP_TRY(rocprim::unique(c->id_tmp.p, tb, srt, vals, nu, (size_t)T, rocprim::equal_to<unsigned long long>(),
+                                s));
+        const uint32_t wgrid = (uint32_t)std::min<uint64_t>((T + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_ids_starts, dim3(wgrid), dim3(256), 0, s, vals, (const uint64_t *)nu,
+                           (uint64_t *)c->id_start.p);
+        hipLaunchKernelGGL(k_ids_write, dim3(wgrid), dim3(256), 0, s, vals, (const uint64_t *)nu,
+                           (const uint64_t *)c->id_start.p, (const uint64_t *)c->idoff.p, q0, d_ids, counts);
+        HIP_TRY(hipGetLastError());
+        q0 = q1;
+    }
+    return AB_OK;
+}

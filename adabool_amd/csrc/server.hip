@@ -14,7 +14,8 @@
 //   run      one ab_query_batch_packed call per batch on the slot's context:
 //            per-query dispatch, one persistent union launch, ids written
 //            straight into each request's reservation; counts and paths come
-//            back through pinned memory behind one event;
+//            back through pinned memory behind one event (ids_only: one
+//            ab_query_batch_ids call, no bitmaps);
 //   deliver  deliver 0: ids stay in device memory until released;
 //            deliver 1: ids are copied to a pinned host mirror of the arena;
 //   release  ab_server_release returns the reservation.
@@ -159,19 +160,6 @@ struct ab_server {
 
 namespace {
 
-uint64_t id_upper_bound(const ab_index *ix, const uint32_t *keys, uint32_t n)
-{
-    uint64_t ub = 0;
-    const uint32_t *seg = ix->seg8.data();
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t k = keys[i];
-        if (k >= ix->nkeys) continue;
-        ub += seg[((uint64_t)k + 1) * kSegStride + 7] - seg[(uint64_t)k * kSegStride + 7];
-        if (ub >= ix->nvalues) return ix->nvalues;
-    }
-    return ub;
-}
-
 // Worker: admission, batch formation, launch, completion.  Holds the lock
 // except while it talks to the device.
 void serve(ab_server *s)
@@ -302,13 +290,18 @@ void serve(ab_server *s)
                             memcpy(hk + sl.qoff[i], sl.members[i]->keys.data(), sl.members[i]->keys.size() * 4);
                     hipStream_t str = sl.ctx->stream;
                     if (K) e = hipMemcpyAsync(sl.dkeys.p, sl.hkeys.p, K * 4, hipMemcpyHostToDevice, str);
-                    if (e == hipSuccess)
+                    if (e == hipSuccess && s->cfg.ids_only) {
+                        st = ab_query_batch_ids(sl.ctx, (const uint32_t *)sl.dkeys.p, sl.qoff.data(), (uint32_t)nq,
+                                                (uint32_t *)s->arena.p, sl.idoff.data(), (uint32_t *)sl.dcounts.p);
+                        memset(sl.hpaths.p, AB_PATH_IDS, nq);
+                    } else if (e == hipSuccess) {
                         st = ab_query_batch_packed(sl.ctx, (const uint32_t *)sl.dkeys.p, sl.qoff.data(), (uint32_t)nq,
                                                    s->cfg.path, nullptr, (uint32_t *)s->arena.p, sl.idoff.data(),
                                                    (uint32_t *)sl.dcounts.p, nullptr);
+                    }
                     if (e == hipSuccess && st == AB_OK)
                         e = hipMemcpyAsync(sl.hcounts.p, sl.dcounts.p, nq * 4, hipMemcpyDeviceToHost, str);
-                    if (e == hipSuccess && st == AB_OK)
+                    if (e == hipSuccess && st == AB_OK && !s->cfg.ids_only)
                         e = hipMemcpyAsync(sl.hpaths.p, sl.ctx->sel.p, nq, hipMemcpyDeviceToHost, str);
                     if (e == hipSuccess && st == AB_OK) e = hipEventRecord(sl.ev, str);
                 }
@@ -363,6 +356,7 @@ extern "C" ab_status ab_server_config_default(ab_server_config *cfg)
     cfg->path = AB_PATH_AUTO;
     cfg->deliver = 0;
     cfg->batch_mode = 1;
+    cfg->ids_only = 0;
     return AB_OK;
 }
 
@@ -374,7 +368,8 @@ extern "C" ab_status ab_server_create(ab_index *ix, const ab_server_config *cfg,
     }
     if (cfg->max_batch < 1 || cfg->max_batch > 4096 || cfg->inflight < 1 || cfg->inflight > 8 ||
         cfg->arena_bytes < 4096 || cfg->deliver < 0 || cfg->deliver > 1 || cfg->path < 0 ||
-        cfg->path > AB_PATH_LEARNED || cfg->batch_mode < 0 || cfg->batch_mode > 2) {
+        cfg->path > AB_PATH_LEARNED || cfg->batch_mode < 0 || cfg->batch_mode > 2 ||
+        cfg->ids_only < 0 || cfg->ids_only > 1) {
         set_error("ab_server_create: config out of range");
         return AB_INVALID_ARG;
     }

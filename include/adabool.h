@@ -49,7 +49,8 @@ typedef enum ab_path {
     AB_PATH_DENSE = 1,     /* dense bitmap-probe over the value-tiled key stream */
     AB_PATH_SPARSE = 2,    /* sparse load-balanced list union */
     AB_PATH_THRESHOLD = 3, /* dense iff M(q) >= tau * R */
-    AB_PATH_LEARNED = 4    /* MLP cost model; falls back to threshold without one */
+    AB_PATH_LEARNED = 4,   /* MLP cost model; falls back to threshold without one */
+    AB_PATH_IDS = 5        /* id-only output (ab_query_batch_ids); reported, never requested */
 } ab_path;
 
 /* Record types of the compressed lists (src/AdaBool/inverted_list.h:16-26). */
@@ -212,7 +213,8 @@ enum {
     AB_K_POPCOUNT = 6,
     AB_K_WRITE_IDS = 7,
     AB_K_FUSED_UNION = 8, /* persistent dense + sparse union (device-dispatched batches) */
-    AB_K_NCLASSES = 9
+    AB_K_IDS = 9,         /* id-only batch: expansion, sort, unique, write (ab_query_batch_ids) */
+    AB_K_NCLASSES = 10
 };
 /* Brackets every kernel class of later calls with HIP events on the context
  * stream (off by default; on costs a few microseconds per call). */
@@ -287,6 +289,30 @@ ab_status ab_query_batch_packed(ab_ctx *ctx, const uint32_t *d_keys, const uint6
                                 uint32_t nq, int path, uint32_t *d_bitmaps, uint32_t *d_ids,
                                 const uint64_t *h_id_off, uint32_t *d_counts, uint8_t *h_paths);
 
+/* Id-only batch: query i's ascending ids at d_ids + h_id_off[i] (u32 units)
+ * and its count at d_counts[i] (NULL: not returned), formed without any
+ * result bitmap.  The query's list records are expanded into values, then
+ * radix-sorted and deduplicated on the device, so memory and traffic follow
+ * the values its lists hold, not nvalues: it serves indices whose per-query
+ * bitmaps would not fit (nvalues up to 2^32 - 1) and queries whose results
+ * are a small part of nvalues.  Query i needs room for ab_query_id_bound ids;
+ * ranges must not overlap.  The batch runs in parts of at most the context's
+ * id budget of expanded values (ab_ctx_set_id_budget; a larger query forms a
+ * part of its own), with one host round trip per call.  The result equals
+ * ab_query_batch's ids for every query.  No reference counterpart: the
+ * reference always forms a bitmap of nvalues bits per query and collects ids
+ * from it on the host (test/adsfilter_q_all.cu:108-119). */
+ab_status ab_query_batch_ids(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_qoff,
+                             uint32_t nq, uint32_t *d_ids, const uint64_t *h_id_off,
+                             uint32_t *d_counts);
+/* Upper bound of a query's id count from host keys: the sum of its in-range
+ * keys' list lengths (repeats included), capped at nvalues. */
+ab_status ab_query_id_bound(const ab_index *ix, const uint32_t *h_keys, uint32_t n,
+                            uint64_t *bound);
+/* Expanded values per part of ab_query_batch_ids (>= 4096; default 2^27,
+ * 3 GiB of scratch). */
+ab_status ab_ctx_set_id_budget(ab_ctx *ctx, uint64_t values);
+
 /* Bitmap -> ascending ids for nq bitmaps at d_bitmaps + i * stride
  * (CollectIdsFromBitmap, test/adsfilter_q_all.cu:108-119, on the device). */
 ab_status ab_compact_ids(ab_ctx *ctx, const uint32_t *d_bitmaps, uint32_t nq,
@@ -354,11 +380,12 @@ typedef struct ab_server_config {
     int path;             /* dispatch of every request (default AB_PATH_AUTO) */
     int deliver;          /* 0: ids in device memory; 1: ids copied to a pinned host mirror */
     int batch_mode;       /* ab_ctx_set_batch_mode of the contexts (default 1) */
+    int ids_only;         /* 1: batches run ab_query_batch_ids (no bitmaps; path AB_PATH_IDS) */
 } ab_server_config;
 typedef struct ab_result {
     uint64_t ticket;
     int status;           /* AB_OK, or AB_OOM when the request can never fit the arena */
-    int path;             /* path taken (AB_PATH_DENSE / AB_PATH_SPARSE) */
+    int path;             /* path taken (AB_PATH_DENSE / AB_PATH_SPARSE / AB_PATH_IDS) */
     uint32_t count;       /* ids */
     const uint32_t *ids;  /* device (deliver 0) or host (deliver 1) pointer, valid until release */
     double latency_ms;    /* submit -> results ready */

@@ -41,6 +41,7 @@ def lib():
     sig = {
         "or_csr_build": (C.c_int, [_VP, _VP, C.c_uint64, C.c_uint32, _VP, _VP]),
         "or_union_csr": (C.c_uint64, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, C.c_uint32, _VP, _VP]),
+        "or_union_ids": (C.c_uint64, [_VP, _VP, C.c_uint32, _VP, C.c_uint32, _VP]),
         "or_scan_coo": (C.c_uint64, [_VP, _VP, C.c_uint64, _VP, C.c_uint32, C.c_uint32, _VP]),
         "or_collect_ids": (C.c_uint64, [_VP, C.c_uint64, C.c_uint32, _VP]),
         "or_ref_index_build": (_VP, [_VP, _VP, C.c_uint64, C.c_int]),
@@ -106,6 +107,14 @@ class CSR:
         n = lib().or_union_csr(_p(self.off), _p(self.vals), self.nkeys, _p(qa), qa.size, self.nvalues, _p(bm),
                                _p(ids))
         return bm, ids[:n].copy()
+
+    def union_ids(self, q) -> np.ndarray:
+        """Ascending ids of query q, by gather + sort + dedupe (no bitmap)."""
+        qa = _u32(q)
+        room = sum(int(self.off[k + 1] - self.off[k]) for k in qa.tolist() if k < self.nkeys)
+        ids = np.zeros(max(room, 1), np.uint32)
+        n = lib().or_union_ids(_p(self.off), _p(self.vals), self.nkeys, _p(qa), qa.size, _p(ids))
+        return ids[:n].copy()
 
     def union_batch(self, qkeys, qoff, nthreads: int = 1):
         """Multi-threaded union of many queries: (wall seconds, counts, latency ms, checksum)."""

@@ -83,6 +83,23 @@ static int cmp_u32(const void *a, const void *b)
     return (x > y) - (x < y);
 }
 
+/* test/adsfilter_q_all.cu:196-208 -- the union as a sorted set of values. */
+uint64_t or_union_ids(const uint64_t *off, const uint32_t *vals, uint32_t nkeys,
+                      const uint32_t *q, uint32_t n, uint32_t *ids)
+{
+    uint64_t m = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t k = q[i];
+        if (k >= nkeys) continue;
+        for (uint64_t j = off[k]; j < off[k + 1]; ++j) ids[m++] = vals[j];
+    }
+    qsort(ids, m, sizeof(uint32_t), cmp_u32);
+    uint64_t u = 0;
+    for (uint64_t i = 0; i < m; ++i)
+        if (u == 0 || ids[u - 1] != ids[i]) ids[u++] = ids[i];
+    return u;
+}
+
 /* test/adsfilter_q_all.cu:196-208 -- full COO scan with a query key set. */
 uint64_t or_scan_coo(const uint32_t *keys, const uint32_t *vals, uint64_t nnz,
                      const uint32_t *q, uint32_t n, uint32_t nvalues,

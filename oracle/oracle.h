@@ -21,6 +21,10 @@
  *                          algorithm: key-stream scan + emit vs. CSR walk).
  *   or_sparse_union     -- equals or_union_csr (extent lookup + LBS chunking
  *                          + emit vs. CSR walk).
+ *   or_union_ids        -- equals or_union_csr's ids on random inputs
+ *                          (gather + sort + dedupe vs. bitmap walk) and a
+ *                          pure-Python set union on tiny ones, including
+ *                          values at 2^32 - 2.
  *   or_stats            -- closed forms (median of even/odd n, linear
  *                          interpolation) checked against numpy.percentile.
  *   or_mlp_*            -- finite differences and a closed-form 1-layer case.
@@ -47,6 +51,13 @@ int or_csr_build(const uint32_t *keys, const uint32_t *vals, uint64_t nnz,
 uint64_t or_union_csr(const uint64_t *off, const uint32_t *vals, uint32_t nkeys,
                       const uint32_t *q, uint32_t n, uint32_t nvalues,
                       uint32_t *bitmap, uint32_t *ids);
+
+/* The same set without a bitmap: the lists of q's in-range keys gathered
+ * into ids (room for the sum of their lengths), sorted and deduplicated in
+ * place; returns |R(q)|.  test/adsfilter_q_all.cu:196-208 (predicate),
+ * 108-119 (ascending ids). */
+uint64_t or_union_ids(const uint64_t *off, const uint32_t *vals, uint32_t nkeys,
+                      const uint32_t *q, uint32_t n, uint32_t *ids);
 
 /* Brute-force scan over all pairs (the reference's own ground truth,
  * test/adsfilter_q_all.cu:196-208). */

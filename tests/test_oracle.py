@@ -146,6 +146,7 @@ def test_union_algorithms_agree(seed):
         for step in (3, 256):
             assert np.array_equal(bm, ix.sparse_union(q, nv, step)[0])
         assert np.array_equal(ids, oracle.collect_ids(bm, bm.size * 32, nv))
+        assert np.array_equal(ids, csr.union_ids(q))
         if n <= 400:
             pbm, pids = py_union(keys, vals, q, nv)
             assert np.array_equal(bm, pbm) and np.array_equal(ids, pids)
@@ -162,6 +163,29 @@ def test_union_tiny_by_hand():
     bm, _ = csr.union([0, 2])
     assert bm.size == 4  # align_up(ceil(100/8), 16) = 16 bytes
     assert bm[0] == (1 << 3) | (1 << 7) | (1 << 31) and bm[1] == 1 << 8
+
+
+def test_union_ids_by_hand_and_at_the_top_of_u32():
+    """The bitmap-free union: hand-checked answers, repeated keys and pairs,
+    and values next to 2^32 - 1, where a bitmap would take 512 MB."""
+    keys = [0, 0, 1, 2, 2, 5, 0]
+    vals = [3, 40, 3, 7, 31, 64, 3]  # (0, 3) twice
+    csr = oracle.CSR(keys, vals, 6, 100)
+    assert csr.union_ids([0]).tolist() == [3, 40]
+    assert csr.union_ids([2, 1, 2]).tolist() == [3, 7, 31]
+    assert csr.union_ids([5, 5, 99]).tolist() == [64]
+    assert csr.union_ids([]).tolist() == []
+    top = 2 ** 32 - 2
+    rng = np.random.default_rng(5)
+    keys = rng.integers(0, 20, 500).astype(np.uint32)
+    vals = (top - rng.integers(0, 3000, 500)).astype(np.uint32)
+    vals[:3] = [top, 0, 255]
+    csr = oracle.CSR(keys, vals, 20, top + 1)
+    for _ in range(10):
+        q = rng.integers(0, 25, int(rng.integers(0, 12))).astype(np.uint32)
+        qs = set(q.tolist())
+        want = sorted({int(v) for k, v in zip(keys.tolist(), vals.tolist()) if k in qs})
+        assert csr.union_ids(q).tolist() == want
 
 
 def test_decode_equals_pairs():
