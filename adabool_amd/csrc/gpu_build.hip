@@ -6,9 +6,9 @@
 //      exclusive scan numbers the records, another (six counters wide) ranks
 //      each record within its type (GroupKeys, src/AdaBool/inverted_list.h:
 //      390-470: types by window size <=1, <=5, <=9, <=17, <=33, else BITSET);
-//   3. one thread per record writes its header and payload; per-key counts go
-//      into seg8 rows by atomics and an inclusive scan of 8-wide rows turns
-//      them into the key_segments_ prefixes;
+//   3. one thread per record writes its header and payload; the seg8 rows
+//      (key_segments_ prefixes) are the type ranks at each key's first
+//      record, written without atomics;
 //   4. value-tiled copy: per type a stable radix sort of records by tile, a
 //      gather, then blocks of 1024 records per (tile, type), their delta
 //      widths (the ComputeKeyBlockMeta rule) and byte offsets by scans, and
@@ -47,22 +47,11 @@ namespace {
 struct Cnt6 {
     uint32_t c[6];
 };
-struct Row8 {
-    uint32_t c[8];
-};
 struct AddCnt6 {
     __host__ __device__ Cnt6 operator()(const Cnt6 &a, const Cnt6 &b) const
     {
         Cnt6 r;
         for (int i = 0; i < 6; ++i) r.c[i] = a.c[i] + b.c[i];
-        return r;
-    }
-};
-struct AddRow8 {
-    __host__ __device__ Row8 operator()(const Row8 &a, const Row8 &b) const
-    {
-        Row8 r;
-        for (int i = 0; i < 8; ++i) r.c[i] = a.c[i] + b.c[i];
         return r;
     }
 };
@@ -87,6 +76,7 @@ inline uint32_t grid_for(uint64_t n, uint32_t bs = 256)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 1u << 20));
 }
 
+// largest key and value; one atomic per wave
 __global__ void k_max_pairs(const uint32_t *k, const uint32_t *v, uint64_t n, uint32_t *out)
 {
     uint32_t a = 0, b = 0;
@@ -94,8 +84,15 @@ __global__ void k_max_pairs(const uint32_t *k, const uint32_t *v, uint64_t n, ui
         a = max(a, k[i]);
         b = max(b, v[i]);
     }
-    atomicMax(&out[0], a);
-    atomicMax(&out[1], b);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        a = max(a, (uint32_t)__shfl_xor(a, d, 64));
+        b = max(b, (uint32_t)__shfl_xor(b, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], a);
+        atomicMax(&out[1], b);
+    }
 }
 
 __global__ void k_pack(const uint32_t *k, const uint32_t *v, uint64_t n, int vbits, uint64_t *kv)
@@ -133,9 +130,8 @@ struct TypeOut {
     uint32_t *rkey[6];
 };
 
-// header, payload and key of every record; per-key counts into seg8 rows k+1
-__global__ void k_rec_fill(Packed p, const uint32_t *start, const Cnt6 *rank, uint32_t R, uint32_t n, TypeOut o,
-                           uint32_t *seg8)
+// header, payload and key of every record
+__global__ void k_rec_fill(Packed p, const uint32_t *start, const Cnt6 *rank, uint32_t R, uint32_t n, TypeOut o)
 {
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < R; r += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t s = start[r], e = r + 1 < R ? start[r + 1] : n, cnt = e - s;
@@ -158,10 +154,31 @@ __global__ void k_rec_fill(Packed p, const uint32_t *start, const Cnt6 *rank, ui
             uint8_t *dst = o.pay[t] + (uint64_t)idx * pb;
             for (uint32_t b = 1; b < cnt; ++b) dst[b - 1] = (uint8_t)(p.val(s + b) & 0xffu);
         }
-        uint32_t *row = seg8 + ((uint64_t)key + 1) * kSegStride;
-        atomicAdd(&row[t], 1u);
-        atomicAdd(&row[6], 1u);
-        atomicAdd(&row[7], cnt);
+    }
+}
+
+// seg8 rows without atomics: records are sorted by key, so row k (the counts
+// of keys < k) is the state before the first record whose key is >= k: its
+// six type ranks, its record index and its first pair index.  The thread of
+// record r writes the rows of keys (key of record r-1, key of record r]; the
+// thread of r = R writes the rows after the last key, up to row nkeys, with
+// the totals.
+__global__ void k_seg8_rows(Packed p, const uint32_t *start, const Cnt6 *rank, uint32_t R, uint32_t n, Cnt6 tot,
+                            uint32_t nkeys, uint32_t *seg8)
+{
+    for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= R; r += (uint64_t)gridDim.x * blockDim.x) {
+        const int64_t kcur = r < R ? (int64_t)p.key(start[r]) : (int64_t)nkeys;
+        const int64_t kprev = r > 0 ? (int64_t)p.key(start[r - 1]) : -1;
+        if (kcur == kprev) continue;
+        uint32_t row[8];
+        for (int t = 0; t < 6; ++t) row[t] = r < R ? rank[r].c[t] : tot.c[t];
+        row[6] = (uint32_t)r;
+        row[7] = r < R ? start[r] : n;
+        for (int64_t k = kprev + 1; k <= kcur; ++k) {
+            uint4 *dst = reinterpret_cast<uint4 *>(seg8 + (uint64_t)k * kSegStride);
+            dst[0] = make_uint4(row[0], row[1], row[2], row[3]);
+            dst[1] = make_uint4(row[4], row[5], row[6], row[7]);
+        }
     }
 }
 
@@ -448,18 +465,15 @@ extern "C" ab_status ab_index_build_device(const uint32_t *d_keys, const uint32_
     const size_t seg_words = ((size_t)nkeys + 1) * kSegStride;
     AB_TRY(resident(ix.get(), seg_words * 4, &seg));
     d.seg8 = (const uint32_t *)seg;
-    if (R) {
+    if (R)
         hipLaunchKernelGGL(k_rec_fill, dim3(grid_for(R)), dim3(256), 0, s, P, start.as<uint32_t>(), rank.as<Cnt6>(), R,
-                           (uint32_t)n, o, (uint32_t *)seg);
-        size_t need = 0;
-        Row8 *rows = (Row8 *)seg;
-        Tmp scanned;
-        HIP_TRY(scanned.alloc(seg_words * 4));
-        HIP_TRY(rocprim::inclusive_scan(nullptr, need, rows, scanned.as<Row8>(), (size_t)nkeys + 1, AddRow8(), s));
-        Tmp t3;
-        HIP_TRY(t3.alloc(need));
-        HIP_TRY(rocprim::inclusive_scan(t3.p, need, rows, scanned.as<Row8>(), (size_t)nkeys + 1, AddRow8(), s));
-        HIP_TRY(hipMemcpyAsync(seg, scanned.p, seg_words * 4, hipMemcpyDeviceToDevice, s));
+                           (uint32_t)n, o);
+    {
+        Cnt6 totc{};
+        for (int t = 0; t < 6; ++t) totc.c[t] = (uint32_t)tot[t];
+        hipLaunchKernelGGL(k_seg8_rows, dim3(grid_for((uint64_t)R + 1)), dim3(256), 0, s, P, start.as<uint32_t>(),
+                           rank.as<Cnt6>(), R, (uint32_t)n, totc, nkeys, (uint32_t *)seg);
+        HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(s));
     }
     start.reset();
