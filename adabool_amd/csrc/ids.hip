@@ -13,9 +13,10 @@
 //      scan over chunks places each chunk's values;
 //   3. the same chunks again: a work-group scan of the record counts places
 //      every record, whose values go out as (query << 32 | value);
-//   4. radix sort and unique of the packed values (rocPRIM);
-//   5. the first index of every query's run, then each value to the caller's
-//      offset for its query, and the run length as its count.
+//   4. radix sort of the packed values (rocPRIM);
+//   5. first copies flagged and ranked by a scan; the first index of every
+//      query's run; each first copy to the caller's offset for its query, and
+//      the number of first copies as its count.
 // The result equals the bitmap paths' ids for every query: the set union of
 // its in-range keys' lists, ascending (test/adsfilter_q_all.cu:108-119,
 // 196-208).
@@ -54,6 +55,7 @@ constexpr uint32_t kIdsNT = 256;      // threads per work-group
 constexpr uint32_t kIdsChunk = 4096;  // records per chunk
 constexpr uint32_t kIdsWin = 1024;    // key offsets per LDS window
 constexpr uint32_t kIdsGrid = 8192;   // work-groups of the chunk kernels (grid-stride)
+constexpr uint32_t kIdsStage = 16384; // LDS bytes staging one round's values
 
 // Exclusive work-group scan of one u32 per thread; *total = the sum.
 __device__ __forceinline__ uint32_t block_excl_u32(uint32_t x, uint32_t *sw, uint32_t *total)
@@ -178,13 +180,14 @@ __device__ __forceinline__ void rec_values(const Rec &rc, F &&f)
 // Records [c0, c1) of a part, in rounds of kIdsNT (one record per thread).
 // koff: the part's record prefix over its K keys.  EMIT = false: returns the
 // chunk's value count; EMIT = true: writes the chunk's values from out[base]
-// on, in record order, as (query - q0) << 32 | value (never at or past limit).
-template <bool EMIT>
+// on, in record order, as (query - q0) << vbits | value (never at or past
+// limit).  KT: u32 when query and value bits fit, else u64.
+template <bool EMIT, class KT>
 __device__ uint64_t ids_chunk(const DevIndex &ix, const uint32_t *__restrict__ keys,
                               const uint32_t *__restrict__ kq, const uint64_t *__restrict__ koff, uint64_t K,
-                              uint64_t c0, uint64_t c1, uint32_t q0, unsigned long long *__restrict__ out,
-                              uint64_t base, uint64_t limit, unsigned long long *skoff, unsigned long long *s_lo,
-                              unsigned long long *s_hi, uint32_t *sw)
+                              uint64_t c0, uint64_t c1, uint32_t q0, uint32_t vbits, KT *__restrict__ out,
+                              uint64_t base, uint64_t limit, KT *stage, unsigned long long *skoff,
+                              unsigned long long *s_lo, unsigned long long *s_hi, uint32_t *sw)
 {
     uint64_t jw = wg_find<kIdsNT>(koff, K, c0, s_lo, s_hi);
     uint64_t run = 0;
@@ -212,13 +215,28 @@ __device__ uint64_t ids_chunk(const DevIndex &ix, const uint32_t *__restrict__ k
             }
             uint32_t tot;
             const uint32_t ex = block_excl_u32(cnt, sw, &tot);
-            if (EMIT && it < whi) {
-                uint64_t pos = base + run + ex;
-                const unsigned long long qk = (unsigned long long)q << 32;
-                rec_values(rc, [&](uint32_t v) {
-                    if (pos < limit) out[pos] = qk | v;
-                    ++pos;
-                });
+            if constexpr (EMIT) {
+                const KT qk = (KT)q << vbits;
+                if (tot <= kIdsStage / sizeof(KT)) {
+                    // through LDS, so that the round's values leave in
+                    // consecutive, coalesced stores
+                    if (it < whi) {
+                        uint32_t p = ex;
+                        rec_values(rc, [&](uint32_t v) { stage[p++] = qk | (KT)v; });
+                    }
+                    __syncthreads();
+                    for (uint32_t i = threadIdx.x; i < tot; i += kIdsNT) {
+                        const uint64_t pos = base + run + i;
+                        if (pos < limit) out[pos] = stage[i];
+                    }
+                    __syncthreads();
+                } else if (it < whi) {
+                    uint64_t pos = base + run + ex;
+                    rec_values(rc, [&](uint32_t v) {
+                        if (pos < limit) out[pos] = qk | (KT)v;
+                        ++pos;
+                    });
+                }
             }
             run += tot;
         }
@@ -241,24 +259,28 @@ __global__ __launch_bounds__(kIdsNT) void k_ids_count(DevIndex ix, const uint32_
     const uint64_t nch = (W + kIdsChunk - 1) / kIdsChunk;
     for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
         const uint64_t c0 = c * kIdsChunk, c1 = (c0 + kIdsChunk < W) ? c0 + kIdsChunk : W;
-        const uint64_t n = ids_chunk<false>(ix, keys, kq, koff, K, c0, c1, 0, nullptr, 0, 0, skoff, &s_lo, &s_hi, sw);
+        const uint64_t n = ids_chunk<false, uint32_t>(ix, keys, kq, koff, K, c0, c1, 0, 0, nullptr, 0, 0, nullptr, skoff,
+                                                     &s_lo, &s_hi, sw);
         if (threadIdx.x == 0) ccnt[c] = n;
     }
 }
 
+template <class KT>
 __global__ __launch_bounds__(kIdsNT) void k_ids_expand(DevIndex ix, const uint32_t *__restrict__ keys,
                                                        const uint32_t *__restrict__ kq,
                                                        const uint64_t *__restrict__ koff, uint64_t K, uint64_t W,
                                                        const uint64_t *__restrict__ cbase, uint32_t q0,
-                                                       unsigned long long *__restrict__ out, uint64_t limit)
+                                                       uint32_t vbits, KT *__restrict__ out, uint64_t limit)
 {
     __shared__ unsigned long long skoff[kIdsWin + 1];
     __shared__ unsigned long long s_lo, s_hi;
     __shared__ uint32_t sw[kIdsNT / 64];
+    __shared__ __attribute__((aligned(16))) KT stage[kIdsStage / sizeof(KT)];
     const uint64_t nch = (W + kIdsChunk - 1) / kIdsChunk;
     for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
         const uint64_t c0 = c * kIdsChunk, c1 = (c0 + kIdsChunk < W) ? c0 + kIdsChunk : W;
-        (void)ids_chunk<true>(ix, keys, kq, koff, K, c0, c1, q0, out, cbase[c], limit, skoff, &s_lo, &s_hi, sw);
+        (void)ids_chunk<true, KT>(ix, keys, kq, koff, K, c0, c1, q0, vbits, out, cbase[c], limit, stage, skoff,
+                                   &s_lo, &s_hi, sw);
     }
 }
 
@@ -268,29 +290,41 @@ __global__ void k_ids_zero(uint32_t *p, uint32_t n)
     if (i < n) p[i] = 0;
 }
 
-// start[q] = index of query q's first value in the sorted unique array u.
-__global__ void k_ids_starts(const unsigned long long *__restrict__ u, const uint64_t *__restrict__ nu,
-                             uint64_t *__restrict__ start)
+// f[i] = 1 where the sorted packed value x[i] differs from x[i - 1]: the
+// first copy of a value within its query.
+template <class KT>
+__global__ void k_ids_flags(const KT *__restrict__ x, uint64_t n, uint64_t *__restrict__ f)
 {
-    const uint64_t n = *nu;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        f[i] = (i == 0 || x[i] != x[i - 1]) ? 1u : 0u;
+}
+
+// start[q] = index of query q's first element in the sorted array x.
+template <class KT>
+__global__ void k_ids_starts(const KT *__restrict__ x, uint64_t n, uint32_t vbits, uint64_t *__restrict__ start)
+{
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t q = (uint32_t)(u[i] >> 32);
-        if (i == 0 || (uint32_t)(u[i - 1] >> 32) != q) start[q] = i;
+        const uint32_t q = (uint32_t)(x[i] >> vbits);
+        if (i == 0 || (uint32_t)(x[i - 1] >> vbits) != q) start[q] = i;
     }
 }
 
-// ids[id_off[q0 + q] + i - start[q]] = value; counts[q0 + q] = run length.
-__global__ void k_ids_write(const unsigned long long *__restrict__ u, const uint64_t *__restrict__ nu,
+// ex: exclusive scan of the first-copy flags.  A first copy goes to
+// ids[id_off[q0 + q] + ex[i] - ex[start[q]]]; the last element of a query's
+// run writes its count.
+template <class KT>
+__global__ void k_ids_write(const KT *__restrict__ x, uint64_t n, uint32_t vbits, const uint64_t *__restrict__ ex,
                             const uint64_t *__restrict__ start, const uint64_t *__restrict__ id_off, uint32_t q0,
                             uint32_t *__restrict__ ids, uint32_t *__restrict__ counts)
 {
-    const uint64_t n = *nu;
+    const KT vmask = (KT)(((unsigned long long)1 << vbits) - 1);
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const unsigned long long x = u[i];
-        const uint32_t q = (uint32_t)(x >> 32);
-        const uint64_t s = start[q];
-        ids[id_off[q0 + q] + (i - s)] = (uint32_t)x;
-        if (i + 1 == n || (uint32_t)(u[i + 1] >> 32) != q) counts[q0 + q] = (uint32_t)(i + 1 - s);
+        const KT v = x[i];
+        const uint32_t q = (uint32_t)(v >> vbits);
+        const uint64_t base = ex[start[q]];
+        const bool first = i == 0 || v != x[i - 1];
+        if (first) ids[id_off[q0 + q] + (ex[i] - base)] = (uint32_t)(v & vmask);
+        if (i + 1 == n || (uint32_t)(x[i + 1] >> vbits) != q) counts[q0 + q] = (uint32_t)(ex[i] + first - base);
     }
 }
 
@@ -302,6 +336,37 @@ int bits_for(uint64_t x)
         x >>= 1;
     }
     return b;
+}
+
+// Expansion, sort and write of one part whose chunk value counts are scanned
+// into id_cbase.
+template <class KT>
+ab_status sort_and_write(ab_ctx *c, const DevIndex &d, const uint32_t *keys_p, const uint32_t *kq_p, uint64_t Kp,
+                         uint64_t W, uint32_t grid, uint32_t q0, int vbits, int end_bit, uint64_t T, uint32_t *d_ids,
+                         uint32_t *counts)
+{
+    hipStream_t s = c->stream;
+    KT *vals = (KT *)c->id_vals.p, *srt = (KT *)c->id_sorted.p;
+    hipLaunchKernelGGL(k_ids_expand<KT>, dim3(grid), dim3(kIdsNT), 0, s, d, keys_p, kq_p,
+                       (const uint64_t *)c->koff.p, Kp, W, (const uint64_t *)c->id_cbase.p, q0, (uint32_t)vbits, vals,
+                       T);
+    HIP_TRY(hipGetLastError());
+    size_t tb = 0;
+    HIP_TRY(rocprim::radix_sort_keys(nullptr, tb, vals, srt, (size_t)T, 0, end_bit, s));
+    AB_TRY(c->id_tmp.ensure(tb + 64));
+    HIP_TRY(rocprim::radix_sort_keys(c->id_tmp.p, tb, vals, srt, (size_t)T, 0, end_bit, s));
+    // first copies flagged, then ranked by a scan (the expansion buffer holds
+    // the flags, then their exclusive scan)
+    const uint32_t wgrid = (uint32_t)std::min<uint64_t>((T + 255) / 256, 16384);
+    uint64_t *ex = (uint64_t *)c->id_vals.p;
+    hipLaunchKernelGGL(k_ids_flags<KT>, dim3(wgrid), dim3(256), 0, s, srt, T, ex);
+    launch_scan_u64(ex, ex, T, (uint64_t *)c->parts.p, s);
+    hipLaunchKernelGGL(k_ids_starts<KT>, dim3(wgrid), dim3(256), 0, s, srt, T, (uint32_t)vbits,
+                       (uint64_t *)c->id_start.p);
+    hipLaunchKernelGGL(k_ids_write<KT>, dim3(wgrid), dim3(256), 0, s, srt, T, (uint32_t)vbits, (const uint64_t *)ex,
+                       (const uint64_t *)c->id_start.p, (const uint64_t *)c->idoff.p, q0, d_ids, counts);
+    HIP_TRY(hipGetLastError());
+    return AB_OK;
 }
 
 }  // namespace
@@ -392,6 +457,7 @@ extern "C" ab_status ab_query_batch_ids(ab_ctx *c, const uint32_t *d_keys, const
 
     // ---- parts of at most id_budget values (a larger query forms its own) ----
     const uint64_t budget = c->id_budget;
+    const int vbits = std::max(1, bits_for(ix->nvalues ? ix->nvalues - 1 : 0));
     uint32_t q0 = 0;
     while (q0 < nq) {
         uint64_t T = tot[2ull * q0 + 1], W = tot[2ull * q0];
@@ -409,7 +475,7 @@ extern "C" ab_status ab_query_batch_ids(ab_ctx *c, const uint32_t *d_keys, const
         const uint64_t nch = (W + kIdsChunk - 1) / kIdsChunk;
         // record numbering, chunk value counts, chunk bases
         AB_TRY(c->koff.ensure((Kp + 1) * 8));
-        AB_TRY(c->parts.ensure((scan_parts_needed(std::max(Kp, nch)) + 2) * 8));
+        AB_TRY(c->parts.ensure((scan_parts_needed(std::max(std::max(Kp, nch), T)) + 2) * 8));
         AB_TRY(c->id_cbase.ensure((nch + 1) * 8));
         AB_TRY(c->id_ccnt.ensure(nch * 8 + 8));
         AB_TRY(c->id_vals.ensure(T * 8 + 64));
@@ -422,28 +488,15 @@ extern "C" ab_status ab_query_batch_ids(ab_ctx *c, const uint32_t *d_keys, const
                            (const uint64_t *)c->koff.p, Kp, W, (uint64_t *)c->id_ccnt.p);
         HIP_TRY(hipGetLastError());
         launch_scan_u64((const uint64_t *)c->id_ccnt.p, (uint64_t *)c->id_cbase.p, nch, (uint64_t *)c->parts.p, s);
-        hipLaunchKernelGGL(k_ids_expand, dim3(grid), dim3(kIdsNT), 0, s, d, keys_p, kq_p,
-                           (const uint64_t *)c->koff.p, Kp, W, (const uint64_t *)c->id_cbase.p, q0,
-                           (unsigned long long *)c->id_vals.p, T);
-        HIP_TRY(hipGetLastError());
-        // sort + unique (query-major, values ascending)
-        const int end_bit = 32 + bits_for(q1 - q0 - 1);
-        unsigned long long *vals = (unsigned long long *)c->id_vals.p, *srt = (unsigned long long *)c->id_sorted.p;
-        uint64_t *nu = (uint64_t *)c->id_ccnt.p + nch;  // one u64 after the chunk counts
-        size_t tb = 0, tb2 = 0;
-        HIP_TRY(rocprim::radix_sort_keys(nullptr, tb, vals, srt, (size_t)T, 0, end_bit, s));
-        HIP_TRY(rocprim::unique(nullptr, tb2, srt, vals, nu, (size_t)T, rocprim::equal_to<unsigned long long>(), s));
-        AB_TRY(c->id_tmp.ensure(std::max(tb, tb2) + 64));
-        tb = std::max(tb, tb2);
-        HIP_TRY(rocprim::radix_sort_keys(c->id_tmp.p, tb, vals, srt, (size_t)T, 0, end_bit, s));
-        HIP_TRY(rocprim::unique(c->id_tmp.p, tb, srt, vals, nu, (size_t)T, rocprim::equal_to<unsigned long long>(),
-                                s));
-        const uint32_t wgrid = (uint32_t)std::min<uint64_t>((T + 255) / 256, 16384);
-        hipLaunchKernelGGL(k_ids_starts, dim3(wgrid), dim3(256), 0, s, vals, (const uint64_t *)nu,
-                           (uint64_t *)c->id_start.p);
-        hipLaunchKernelGGL(k_ids_write, dim3(wgrid), dim3(256), 0, s, vals, (const uint64_t *)nu,
-                           (const uint64_t *)c->id_start.p, (const uint64_t *)c->idoff.p, q0, d_ids, counts);
-        HIP_TRY(hipGetLastError());
+        // packed (query - q0) << vbits | value, sorted query-major with values
+        // ascending; u32 when both fit
+        const int qbits = bits_for(q1 - q0 - 1);
+        if (vbits < 32 && vbits + qbits <= 32)
+            AB_TRY(sort_and_write<uint32_t>(c, d, keys_p, kq_p, Kp, W, grid, q0, vbits, vbits + qbits, T, d_ids,
+                                            counts));
+        else
+            AB_TRY(sort_and_write<unsigned long long>(c, d, keys_p, kq_p, Kp, W, grid, q0, vbits, vbits + qbits, T,
+                                                      d_ids, counts));
         q0 = q1;
     }
     return AB_OK;
