@@ -7,9 +7,13 @@ each is timed.  Two label sets, one per model slot:
 * single (MODEL_SINGLE): one synchronized single-query call per path
   (enqueue -> stream synchronize, host wall clock), best of `repeats`;
 * batch (MODEL_BATCH): the query's share of a batch, from HIP-event device
-  times -- sparse: its own zero-fill and union kernels run alone; dense: the
-  tile and mask kernels of a random group of up to 64 queries containing it,
-  divided by the group size (one pass over the index serves the group).
+  times of device-dispatched batches (one fused union launch) over random
+  groups of 128 queries, each group run once on either path: dense -- the
+  group's mask and union kernels divided by the group size (one pass over the
+  index serves the group); sparse -- the zero-fill and plan divided by the
+  group size, plus the union in proportion to the query's records M(q) (the
+  load-balanced union's work).  Timing a sparse query alone would charge it
+  a whole launch that a batch shares.
 
 Each model (12 features -> 64 -> 64 -> 2, log milliseconds of dense and
 sparse) is trained by the library's own HIP kernels (ab_mlp_train); the
@@ -96,28 +100,20 @@ def time_both(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, repea
     return feats, M, t
 
 
-def time_batch_shares(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, seed: int, repeats: int = 2,
-                      group: int = 64):
+def time_batch_shares(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, M: np.ndarray, seed: int,
+                      repeats: int = 2, group: int = 128):
     """Per query: device ms of its share of a batch on each path (see module doc)."""
     import torch
 
     nq = qoff.size - 1
     sw = ix.stride_words
-    dk = torch.from_numpy(keys.view(np.int32)).cuda() if keys.size else torch.zeros(1, dtype=torch.int32,
-                                                                                     device="cuda")
     bm = torch.empty(group * sw, dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
     t = np.full((nq, 2), np.inf)
+    ctx.set_batch_mode(1)
+    ctx.set_dense_group(group)
     ctx.set_timing(True)
     try:
-        base = dk.data_ptr()
-        for i in range(nq):
-            a, b = int(qoff[i]), int(qoff[i + 1])
-            for _ in range(repeats):
-                ctx.timing(reset=True)
-                ctx.query_batch_device(base + 4 * a, np.array([0, b - a], np.uint64), AB_PATH_SPARSE, bm)
-                k = ctx.timing(reset=True)
-                t[i, 1] = min(t[i, 1], k["sparse_union"][0] + k["sparse_zero"][0])
         rng = np.random.default_rng(seed)
         for _ in range(repeats):
             perm = rng.permutation(nq)
@@ -125,16 +121,24 @@ def time_batch_shares(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarra
                 members = perm[g0:g0 + group]
                 gk = np.concatenate([keys[int(qoff[m]):int(qoff[m + 1])] for m in members])
                 go = np.concatenate([[0], np.cumsum([int(qoff[m + 1] - qoff[m]) for m in members])]).astype(np.uint64)
-                gd = torch.from_numpy(gk.view(np.int32)).cuda() if gk.size else dk
+                gd = torch.from_numpy(gk.view(np.int32)).cuda() if gk.size else torch.zeros(1, dtype=torch.int32,
+                                                                                             device="cuda")
                 torch.cuda.synchronize()
                 ctx.timing(reset=True)
                 ctx.query_batch_device(gd, go, AB_PATH_DENSE, bm)
                 k = ctx.timing(reset=True)
-                share = (k["dense_tiles"][0] + k["dense_mask"][0]) / members.size
+                share = (k["fused_union"][0] + k["dense_mask"][0]) / members.size
                 t[members, 0] = np.minimum(t[members, 0], share)
+                ctx.query_batch_device(gd, go, AB_PATH_SPARSE, bm)
+                k = ctx.timing(reset=True)
+                fixed = (k["sparse_zero"][0] + k["sparse_plan"][0]) / members.size
+                w = M[members].astype(np.float64)
+                frac = w / w.sum() if w.sum() > 0 else np.full(members.size, 1.0 / members.size)
+                t[members, 1] = np.minimum(t[members, 1], fixed + k["fused_union"][0] * frac)
     finally:
         ctx.set_timing(False)
-    return t
+    # a query with no records costs its zero-fill only; keep labels positive
+    return np.maximum(t, 1e-6)
 
 
 def fit_threshold(M: np.ndarray, R: np.ndarray, t: np.ndarray) -> float:
@@ -177,6 +181,9 @@ def main(argv: List[str] = None):
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0, help="index size factor (for quick runs)")
+    ap.add_argument("--reuse-data", default="", help="timings file of an earlier run: keep its features, M and "
+                                                     "single-query labels, time the batch shares again")
+    ap.add_argument("--reuse-model", default="", help="with --reuse-data: model file whose single slot is kept")
     a = ap.parse_args(argv)
 
     import torch
@@ -185,15 +192,21 @@ def main(argv: List[str] = None):
     rows = []
     specs = [(filter_spec(int(nv * a.scale), s, k), False) for k, (nv, s) in enumerate(TRAIN_INDICES)]
     specs.append((filter_spec(int(HELDOUT_INDEX[0] * a.scale), HELDOUT_INDEX[1], 99), True))
-    for spec, held in specs:
+    old = dict(np.load(a.reuse_data)) if a.reuse_data else None
+    for n_spec, (spec, held) in enumerate(specs):
         t0 = time.time()
         pk, pv = configs.make_filter(spec)
         ix = Index(pk, pv, spec.nkeys, spec.nvalues)
         del pk, pv
         ctx = Context(ix)
         keys, qoff = query_mix(spec, a.queries, spec.seed + 17)
-        f, M, t1 = time_both(ctx, ix, keys, qoff, a.repeats)
-        tb = time_batch_shares(ctx, ix, keys, qoff, spec.seed, a.repeats)
+        if old is not None:
+            sl = slice(n_spec * a.queries, (n_spec + 1) * a.queries)
+            f, M, t1 = old["features"][sl], old["M"][sl], old["t_single_ms"][sl]
+            assert M.size == qoff.size - 1 and int(old["R"][sl][0]) == ix.info["nrec_total"], "timings file mismatch"
+        else:
+            f, M, t1 = time_both(ctx, ix, keys, qoff, a.repeats)
+        tb = time_batch_shares(ctx, ix, keys, qoff, M, spec.seed, a.repeats)
         R = np.full(M.size, ix.info["nrec_total"], np.uint64)
         rows.append((f, M, R, t1, tb, np.full(M.size, held)))
         print(f"[costmodel] {spec.name}: R={ix.info['nrec_total']} {a.queries} queries timed in "
@@ -216,7 +229,12 @@ def main(argv: List[str] = None):
     ctx = Context(ix)
     out = {"meta": {"trained_on": [list(x) for x in TRAIN_INDICES], "heldout": list(HELDOUT_INDEX),
                     "train_queries": int(tr.sum()), "heldout_queries": int(te.sum())}}
+    kept = json.load(open(a.reuse_model)) if a.reuse_model else None
     for slot, name, T in ((0, "single", T1), (1, "batch", TB)):
+        if kept is not None and name == "single":
+            out[name] = kept[name]
+            print(name, "kept from", a.reuse_model, flush=True)
+            continue
         tau = fit_threshold(M[tr], R[tr], T[tr])
         learned, curve = _train(ctx, slot, F, T, tr, a.epochs)
         thr = M.astype(np.float64) >= tau * R.astype(np.float64)

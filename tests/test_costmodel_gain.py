@@ -64,6 +64,22 @@ def test_learned_beats_threshold_single_query(data):
     assert r_learned < costmodel.regret(np.ones_like(choice), t)
 
 
+def test_batch_rules_beat_one_path_for_all(data):
+    """Batch dispatch, on held-out batch-share labels: the threshold rule and
+    the network are each within 1% of the per-query best, while sending every
+    query dense costs 40% more and every query sparse many times more."""
+    model, d = data
+    t = d["t_batch_ms"].astype(np.float64)
+    learned = _forward(model["batch"], d["features"])
+    choice = learned[:, 0] < learned[:, 1]
+    tau = model["batch"]["meta"]["tau"]
+    thr = d["M"].astype(np.float64) >= tau * d["R"].astype(np.float64)
+    all_dense = costmodel.regret(np.ones_like(choice), t)
+    all_sparse = costmodel.regret(np.zeros_like(choice), t)
+    for r in (costmodel.regret(choice, t), costmodel.regret(thr, t)):
+        assert r < 0.01 and r < all_dense / 10 and r < all_sparse / 10, (r, all_dense, all_sparse)
+
+
 def test_regret_definition():
     t = np.array([[1.0, 2.0], [3.0, 1.0]])
     assert costmodel.regret(np.array([True, False]), t) == 0.0
