@@ -542,13 +542,15 @@ class Context:
 
     # batches --------------------------------------------------------------
     def query_batch_device(self, d_keys, qoff, path: int, d_bitmaps=None, d_ids=None, ids_stride: int = 0,
-                           d_counts=None) -> np.ndarray:
+                           d_counts=None, want_paths: bool = True):
+        """Enqueues one batch; returns each query's path (which waits for the
+        batch), or None with want_paths=False (nothing waits)."""
         off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
         nq = off.size - 1
-        paths = np.zeros(max(nq, 1), np.uint8)
+        paths = np.zeros(max(nq, 1), np.uint8) if want_paths else None
         _check(_lib.ab_query_batch(self._h, _ptr(d_keys), _ptr(off), nq, path, _ptr(d_bitmaps), _ptr(d_ids),
                                    ids_stride, _ptr(d_counts), _ptr(paths)))
-        return paths[:nq]
+        return paths[:nq] if want_paths else None
 
     def autotune(self, d_keys, qoff, path: int = AB_PATH_AUTO, repeats: int = 3) -> list:
         """Times the launch-parameter grid on this batch and keeps the fastest

@@ -147,6 +147,10 @@ def main():
     ap.add_argument("--tile-shift", type=int, default=0, help="value tile of the built index (0: automatic)")
     ap.add_argument("--tile-shifts", default="10,12",
                     help="value-tile sizes (log2) the autotuner also tries, one device-built index each")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="contexts (one stream, graph and output buffers each) that consecutive timed steps "
+                         "alternate over, so one step's compaction runs beside the next one's union; 1: every "
+                         "step waits for the previous one")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -289,12 +293,53 @@ def main():
         ctx.set_timing(False)
         return kt
 
-    wall, step_ms, paths = run(path, args.steps, args.warmup)
+    # ---- more contexts for pipelined steps: same index and settings, own
+    # stream, graph and output buffers
+    ctxs, bufs = [ctx], [(d_bm, d_ids, d_cnt)]
+    for _ in range(max(1, args.streams) - 1):
+        c2 = setup_ctx(ix)
+        if tuned:
+            c2.apply_tuning(tuned)
+        ctxs.append(c2)
+        bufs.append((torch.empty_like(d_bm), torch.empty_like(d_ids), torch.empty_like(d_cnt)))
+    torch.cuda.synchronize()
+
+    def run_streams(p, steps, warmup):
+        """warmup, then `steps` steps enqueued round-robin over the contexts
+        with no wait between them; bracketed by device synchronizations."""
+        for c, (b, i, n) in zip(ctxs, bufs):
+            for _ in range(warmup):
+                c.query_batch_device(d_keys, qo, p, b, i, nv, n, want_paths=False)
+            c.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            b, i, n = bufs[k % len(ctxs)]
+            ctxs[k % len(ctxs)].query_batch_device(d_keys, qo, p, b, i, nv, n, want_paths=False)
+        for c in ctxs:
+            c.sync()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    wall_serial, step_ms, paths = run(path, args.steps, args.warmup)
     counts = d_cnt.cpu().numpy().view(np.uint32).astype(np.int64)
-    qps = nq * args.steps / wall
+    qps_serial = nq * args.steps / wall_serial
     ndense = int((paths == ab.AB_PATH_DENSE).sum())
-    log(f"batched ({dispatch}, mode {args.batch_mode}): {qps:.0f} q/s, {wall / args.steps * 1e3:.2f} ms/step, "
-        f"dense {ndense}/{nq}")
+    log(f"batched ({dispatch}, mode {args.batch_mode}), one step at a time: {qps_serial:.0f} q/s, "
+        f"{wall_serial / args.steps * 1e3:.2f} ms/step, dense {ndense}/{nq}")
+    wall, qps, streams_equal = wall_serial, qps_serial, None
+    if len(ctxs) > 1:
+        for _, i, n in bufs[1:]:
+            n.fill_(-1)
+            i[:nv].fill_(-1)
+        wall = run_streams(path, args.steps, args.warmup)
+        qps = nq * args.steps / wall
+        # every context's outputs equal the serial run's (counts, and the ids of query 0)
+        ids0 = d_ids[: int(counts[0])].cpu()
+        streams_equal = all(np.array_equal(n.cpu().numpy().view(np.uint32).astype(np.int64), counts)
+                            and bool((i[: int(counts[0])].cpu() == ids0).all()) for _, i, n in bufs[1:])
+        log(f"batched ({dispatch}, mode {args.batch_mode}), {len(ctxs)} streams: {qps:.0f} q/s, "
+            f"{wall / args.steps * 1e3:.2f} ms/step; outputs equal the serial run: {streams_equal}")
     ksteps = max(2, min(args.steps, 5))
     kt = kernel_times(path, ksteps)
     log("kernels: " + ", ".join(f"{k}={v[0] / ksteps:.3f}ms/{v[1] // ksteps}" for k, v in kt.items() if v[1]))
@@ -302,8 +347,12 @@ def main():
     nsub = max(2, args.steps // 2)
     if dispatch == "learned":
         tw, tms, tp = run(ab.AB_PATH_THRESHOLD, nsub, 1)
-        thr_cmp = {"qps": round(nq * nsub / tw, 1), "ms_per_step": round(tw / nsub * 1e3, 3),
+        thr_cmp = {"qps_one_step_at_a_time": round(nq * nsub / tw, 1), "ms_per_step": round(tw / nsub * 1e3, 3),
                    "dense_queries": int((tp == ab.AB_PATH_DENSE).sum())}
+        if len(ctxs) > 1:
+            thr_cmp["qps"] = round(nq * args.steps / run_streams(ab.AB_PATH_THRESHOLD, args.steps, 1), 1)
+        else:
+            thr_cmp["qps"] = thr_cmp["qps_one_step_at_a_time"]
         log(f"batched (threshold tau_batch={ctx.batch_threshold:.5f}): {thr_cmp['qps']:.0f} q/s")
     mode_cmp = {}
     for m in (0, 1, 2):
@@ -402,6 +451,9 @@ def main():
                                f"in [8, 20000]; bitmap + ascending ids + counts",
                    "queries_per_step": nq, "dispatch": dispatch, "dense_queries": ndense,
                    "tile_shift": info["tile_shift"]},
+        "streams": len(ctxs),
+        "one_step_at_a_time": {"qps": round(qps_serial, 1), "ms_per_step": round(wall_serial / args.steps * 1e3, 3)},
+        "streams_outputs_equal": streams_equal,
         "latency_ms": {"batch_p50": float(np.percentile(step_ms, 50)), "batch_p99": float(np.percentile(step_ms, 99)),
                        "per_query_threshold_p50": float(np.percentile(lat1, 50)),
                        "per_query_threshold_p99": float(np.percentile(lat1, 99))},
