@@ -93,9 +93,9 @@ def cpu_model() -> str:
 
 def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float):
     """Oracle (plain C, CSR-by-key union -> bitmap + ids) on the host cores,
-    over a bounded prefix of the same queries (one thread per core across
-    queries, and one thread alone); also checks the GPU counts of that prefix
-    against it."""
+    over a bounded sample of the same queries, about budget_s of wall time
+    (a prefix, or whole passes over all of them; one thread per core across
+    queries, and one thread alone); also checks the GPU counts against it."""
     import oracle
 
     nthreads = min(16, os.cpu_count() or 1)
@@ -109,11 +109,19 @@ def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float
     sub = qoff[: ns + 1]
     t, counts, lat, _ = csr.union_batch(keys[: int(sub[-1])], sub, nthreads)
     ok = bool(np.array_equal(counts.astype(np.int64), counts_gpu[:ns].astype(np.int64)))
+    passes = 1
+    while ns == nq and t < budget_s * 0.9:  # every query fits: whole passes until the budget is used
+        tp, cp, lp, _ = csr.union_batch(keys, qoff, nthreads)
+        ok = ok and bool(np.array_equal(cp.astype(np.int64), counts_gpu.astype(np.int64)))
+        t += tp
+        lat = np.concatenate([lat, lp])
+        passes += 1
     # one thread: a smaller prefix of about a quarter of the budget
     n1 = int(min(nq, max(1, budget_s / 4 / (per_q * nthreads))))
     t1, _, lat1, _ = csr.union_batch(keys[: int(qoff[n1])], qoff[: n1 + 1], 1)
-    return {"value": ns / t, "unit": "queries/sec", "cores": nthreads, "kind": "port",
-            "sample": f"first {ns} of the {nq} C2 queries, oracle/oracle.c or_union_batch, {nthreads} threads, "
+    what = f"{passes} passes over all {nq}" if passes > 1 else f"first {ns} of the {nq}"
+    return {"value": ns * passes / t, "unit": "queries/sec", "cores": nthreads, "kind": "port",
+            "sample": f"{what} C2 queries, oracle/oracle.c or_union_batch, {nthreads} threads, "
                       f"{t:.1f} s", "counts_match_gpu": ok, "cpu_model": cpu_model(),
             "host_cpus_visible": os.cpu_count(),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
