@@ -433,6 +433,20 @@ def main():
     base_qps = nb / (time.perf_counter() - t1)
     log(f"per-query threshold: {base_qps:.0f} q/s, p50 {np.percentile(lat1, 50):.3f} ms "
         f"p99 {np.percentile(lat1, 99):.3f} ms; count mismatches vs batch: {mism}")
+    # the same queries one at a time with the single-query cost model
+    learned_pq = None
+    if dispatch == "learned":
+        latl = []
+        tl = time.perf_counter()
+        for i in range(nb):
+            s0 = time.perf_counter()
+            base_ctx.query_device(qk[qo[i]:qo[i + 1]], ab.AB_PATH_LEARNED, d_bm1, d_ids1, d_c1)
+            base_ctx.sync()
+            latl.append((time.perf_counter() - s0) * 1e3)
+        learned_pq = {"qps": round(nb / (time.perf_counter() - tl), 1), "p50_ms": float(np.percentile(latl, 50)),
+                      "p99_ms": float(np.percentile(latl, 99))}
+        log(f"per-query learned: {learned_pq['qps']:.0f} q/s, p50 {learned_pq['p50_ms']:.3f} ms "
+            f"p99 {learned_pq['p99_ms']:.3f} ms")
 
     cpu = None
     if not args.no_cpu:
@@ -466,6 +480,7 @@ def main():
                        "per_query_threshold_p50": float(np.percentile(lat1, 50)),
                        "per_query_threshold_p99": float(np.percentile(lat1, 99))},
         "per_query_threshold_qps": round(base_qps, 1),
+        "per_query_learned": learned_pq,
         "per_query_threshold_tau": base_ctx.threshold,
         "per_query_threshold_tile_shift": base_ctx.index.info["tile_shift"],
         "batched_threshold": thr_cmp,
