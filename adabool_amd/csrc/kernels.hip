@@ -442,7 +442,6 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
                                            uint32_t *tiles, uint32_t tw, uint32_t tile_word0,
                                            typename MaskOf<MODE>::T *col, uint32_t vmask)
 {
-    using M = typename MaskOf<MODE>::T;
     auto flush = [&](uint32_t gw, uint32_t bits) {
         const uint32_t lw = gw - tile_word0;
         if constexpr (MODE == kModeBitmap) {
