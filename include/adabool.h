@@ -281,7 +281,10 @@ ab_status ab_query_batch(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_
 
 /* ab_query_batch with ids packed at caller-chosen offsets: query i's ids go
  * to d_ids + h_id_off[i] (u32 units; ranges must not overlap, and offsets
- * that are multiples of 4 take the 128-bit store path).  Variable-size id
+ * that are multiples of 4 take the 128-bit store path).  Each range must hold
+ * the query's count, which is at most its id bound (ab_query_id_bound); the
+ * library cannot see the size of a device buffer, and a shorter range is
+ * written past.  Variable-size id
  * lists packed on the device, where the reference's batch caller reads every
  * bitmap back and collects each query's ids on the host
  * (test/adsfilter_batch.cu:505-537). */
