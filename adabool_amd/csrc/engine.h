@@ -102,7 +102,9 @@ struct ab_ctx {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         std::vector<uint64_t> key;
+        std::vector<uint64_t> pending;  // the last shape run without a graph
     } gc;
+    ab::DevBuf gkeys;  // graph mode: the batch's keys, copied in before every launch
     uint32_t fused_grid = 0;  // work-groups of the last fused launch
     // per-kernel-class timing
     bool timing = false;

@@ -48,6 +48,12 @@ namespace {
 struct Plan {
     uint32_t nq;
     uint64_t K;
+    // graph mode: the keys are copied to ctx->gkeys before every launch and
+    // the key count is read on the device, so one graph serves every batch
+    // of the same query count whose key count rounds up to the same Kcap
+    bool graph;
+    uint64_t Kcap;
+    const uint32_t *keys;
     uint32_t G;
     int mode;
     bool use_col;
@@ -76,7 +82,7 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
         if (a.path == AB_PATH_DENSE || a.path == AB_PATH_SPARSE) {
             hipLaunchKernelGGL(k_fill_u8, dim3((nq + 255) / 256), dim3(256), 0, s, sel, nq, (uint8_t)a.path);
         } else {
-            launch_features(d, a.d_keys, qoff, nq, ix->R, (float *)c->feat.p, (uint64_t *)c->M.p, s);
+            launch_features(d, p.keys, qoff, nq, ix->R, (float *)c->feat.p, (uint64_t *)c->M.p, s);
             const int slot = nq > 1 ? 1 : 0;
             if ((a.path == AB_PATH_LEARNED || a.path == AB_PATH_AUTO) && c->mlp[slot].loaded) {
                 AB_TRY(mlp_decide(c, slot, (const float *)c->feat.p, nq, sel));
@@ -94,21 +100,23 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     }
     {
         TimedSpan ts(c, AB_K_DENSE_MASK);
-        launch_mask_groups(p.mode, a.d_keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
+        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
                            (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 0, s);
     }
-    if (p.K) {
+    const uint64_t *Kd = p.graph ? qoff + nq : nullptr;  // qoff[nq] = the key count
+    if (p.Kcap) {
         TimedSpan ts(c, AB_K_SPARSE_PLAN);
-        launch_sparse_plan(d, a.d_keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
-        launch_scan_u64((const uint64_t *)c->cnt.p, (uint64_t *)c->koff.p, p.K, (uint64_t *)c->parts.p, s);
+        launch_sparse_plan(d, p.keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
+        launch_scan_u64((const uint64_t *)c->cnt.p, (uint64_t *)c->koff.p, p.Kcap, (uint64_t *)c->parts.p, s, Kd);
     }
     {
         TimedSpan ts(c, AB_K_FUSED_UNION);
         FusedArgs f{};
-        f.keys = a.d_keys;
+        f.keys = p.keys;
         f.kq = (const uint32_t *)c->kq.p;
         f.koff = (const uint64_t *)c->koff.p;
-        f.K = p.K;
+        f.K = p.Kcap ? p.K : 0;
+        f.Kd = p.Kcap ? Kd : nullptr;
         f.chunk = c->sparse_chunk;
         f.plan = (const uint32_t *)c->plan.p;
         f.grp_q = (const uint32_t *)c->grp_q.p;
@@ -126,7 +134,7 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     }
     {
         TimedSpan ts(c, AB_K_DENSE_MASK);
-        launch_mask_groups(p.mode, a.d_keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
+        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
                            (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 1, s);
     }
     if (a.d_ids || a.d_counts) {
@@ -152,6 +160,12 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     Plan p{};
     p.nq = a.nq;
     p.K = a.h_qoff[a.nq];
+    p.graph = c->batch_mode == 2 && !c->timing;
+    p.Kcap = p.K;
+    if (p.graph && p.K) {
+        p.Kcap = 4096;
+        while (p.Kcap < p.K) p.Kcap <<= 1;
+    }
     p.G = std::max<uint32_t>(1, max_group(c));
     p.mode = p.G <= 32 ? kModeMask32 : (p.G <= 64 ? kModeMask64 : kModeMask128);
     p.use_col = col_fits(c, p.mode, p.G);
@@ -170,7 +184,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     const uint64_t nchunk = (p.words + kCompactWords - 1) / kCompactWords;
     std::vector<void *> before;
     for (DevBuf *b : {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan, &c->counter,
-                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm})
+                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm, &c->gkeys})
         before.push_back(b->p);
     p.packed = a.h_id_off && a.d_ids;
     p.stage_bytes = ((size_t)nq + 1) * 8 + (p.packed ? (size_t)nq * 8 : 0);
@@ -183,10 +197,11 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     AB_TRY(c->grp_n.ensure((size_t)ngmax * 4 + 4));
     AB_TRY(c->plan.ensure(16));
     AB_TRY(c->counter.ensure(16));
-    AB_TRY(c->kq.ensure(p.K * 4 + 4));
-    AB_TRY(c->cnt.ensure(p.K * 8 + 8));
-    AB_TRY(c->koff.ensure((p.K + 1) * 8));
-    AB_TRY(c->parts.ensure(scan_parts_needed(p.K) * 8 + 16));
+    AB_TRY(c->kq.ensure(p.Kcap * 4 + 4));
+    AB_TRY(c->cnt.ensure(p.Kcap * 8 + 8));
+    AB_TRY(c->koff.ensure((p.Kcap + 1) * 8));
+    AB_TRY(c->parts.ensure(scan_parts_needed(p.Kcap) * 8 + 16));
+    if (p.graph) AB_TRY(c->gkeys.ensure(p.Kcap * 4 + 16));
     AB_TRY(c->ccnt.ensure((size_t)nq * nchunk * 4 + 4));
     p.bm = a.d_bitmaps;
     if (!p.bm) {
@@ -207,9 +222,10 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     }
     std::vector<void *> after;
     for (DevBuf *b : {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan, &c->counter,
-                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm})
+                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm, &c->gkeys})
         after.push_back(b->p);
     if (before != after) release_graph(c);
+    p.keys = p.graph ? (const uint32_t *)c->gkeys.p : a.d_keys;
 
     // ---- stage the offsets (the graph's copy node reads this pinned area) ----
     AB_TRY(stage_wait(c));
@@ -217,18 +233,25 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     memcpy(c->hstage.p, a.h_qoff, ((size_t)nq + 1) * 8);
     if (p.packed) memcpy((uint8_t *)c->hstage.p + ((size_t)nq + 1) * 8, a.h_id_off, (size_t)nq * 8);
 
-    const bool graph = c->batch_mode == 2 && !c->timing;
-    if (!graph) {
+    if (!p.graph) {
         AB_TRY(enqueue(c, a, p));
     } else {
-        std::vector<uint64_t> key = {(uint64_t)(uintptr_t)a.d_keys, nq, p.K, (uint64_t)a.path, p.grid,
+        // this batch's keys into the buffer the graph reads
+        if (p.K) HIP_TRY(hipMemcpyAsync(c->gkeys.p, a.d_keys, p.K * 4, hipMemcpyDeviceToDevice, s));
+        std::vector<uint64_t> key = {nq, p.Kcap, (uint64_t)a.path, p.grid,
                                      (uint64_t)(uintptr_t)p.bm, p.stride, p.words, (uint64_t)(uintptr_t)a.d_ids,
                                      a.ids_stride, (uint64_t)p.packed, (uint64_t)a.id_off_aligned,
                                      (uint64_t)(uintptr_t)a.d_counts, c->cfg_gen,
                                      (uint64_t)(uintptr_t)c->hstage.p, (uint64_t)(uintptr_t)c->gmask.p,
                                      (uint64_t)(uintptr_t)c->ganybm.p};
         for (void *q : after) key.push_back((uint64_t)(uintptr_t)q);
-        if (!(c->gc.valid && c->gc.key == key)) {
+        const bool hit = c->gc.valid && c->gc.key == key;
+        if (!hit && c->gc.pending != key) {
+            // a shape seen once runs as launches; it is captured when it comes
+            // again in the next call
+            c->gc.pending = key;
+            AB_TRY(enqueue(c, a, p));
+        } else if (!hit) {
             release_graph(c);
             HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             ab_status st = enqueue(c, a, p);
@@ -249,8 +272,10 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
             c->gc.exec = ex;
             c->gc.key = key;
             c->gc.valid = true;
+            HIP_TRY(hipGraphLaunch(c->gc.exec, s));
+        } else {
+            HIP_TRY(hipGraphLaunch(c->gc.exec, s));
         }
-        HIP_TRY(hipGraphLaunch(c->gc.exec, s));
     }
     HIP_TRY(hipEventRecord(c->stage_ev, s));
     c->stage_pending = true;

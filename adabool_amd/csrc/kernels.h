@@ -42,7 +42,10 @@ void launch_zero_qlist(uint32_t *bm, uint64_t stride_words, uint64_t words, cons
                        uint32_t n, hipStream_t s);
 void launch_sparse_plan(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff,
                         const uint8_t *sel, uint32_t nq, uint32_t *kq, uint64_t *cnt, hipStream_t s);
-void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *parts, hipStream_t s);
+// n_dev set: n is only the capacity (grid size) and the item count is read
+// from *n_dev on the device (at most n).
+void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *parts, hipStream_t s,
+                     const uint64_t *n_dev = nullptr);
 void launch_sparse_union(const DevIndex &ix, const uint32_t *keys, const uint32_t *kq,
                          const uint64_t *koff, uint64_t K, uint32_t *out, uint64_t out_stride,
                          uint32_t chunk, uint32_t grid, hipStream_t s);
@@ -73,6 +76,7 @@ struct FusedArgs {
     const uint32_t *keys, *kq;   // sparse: keys, query of each key
     const uint64_t *koff;        // sparse: record prefix over keys (K+1)
     uint64_t K;
+    const uint64_t *Kd;          // when set, K is read here on the device
     uint32_t chunk;              // sparse items per work item
     const uint32_t *plan;        // [0] dense groups
     const uint32_t *grp_q;       // 64 query slots per group

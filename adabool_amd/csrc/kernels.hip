@@ -89,9 +89,11 @@ __device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long
     return before + incl - x;
 }
 
-__global__ __launch_bounds__(kScanNT) void k_scan_tile_sums(const uint64_t *in, uint64_t n, uint64_t *parts)
+__global__ __launch_bounds__(kScanNT) void k_scan_tile_sums(const uint64_t *in, uint64_t n, const uint64_t *n_dev,
+                                                             uint64_t *parts)
 {
     __shared__ unsigned long long sh[kScanNT / 64];
+    if (n_dev) n = *n_dev;
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     unsigned long long s = 0;
     for (uint32_t i = 0; i < kScanIPT; ++i) {
@@ -104,9 +106,10 @@ __global__ __launch_bounds__(kScanNT) void k_scan_tile_sums(const uint64_t *in, 
 }
 
 // single work-group: exclusive scan of parts[0..np) in place; parts[np] = total
-__global__ __launch_bounds__(kScanNT) void k_scan_parts(uint64_t *parts, uint64_t np)
+__global__ __launch_bounds__(kScanNT) void k_scan_parts(uint64_t *parts, uint64_t np, const uint64_t *n_dev)
 {
     __shared__ unsigned long long sh[kScanNT / 64];
+    if (n_dev) np = (*n_dev + kScanTile - 1) / kScanTile;
     unsigned long long carry = 0;
     for (uint64_t base = 0; base < np; base += kScanNT) {
         uint64_t j = base + threadIdx.x;
@@ -118,10 +121,19 @@ __global__ __launch_bounds__(kScanNT) void k_scan_parts(uint64_t *parts, uint64_
     if (threadIdx.x == 0) parts[np] = carry;
 }
 
-__global__ __launch_bounds__(kScanNT) void k_scan_apply(const uint64_t *in, uint64_t n, const uint64_t *parts,
-                                                         uint64_t np, uint64_t *out)
+__global__ __launch_bounds__(kScanNT) void k_scan_apply(const uint64_t *in, uint64_t n, const uint64_t *n_dev,
+                                                         const uint64_t *parts, uint64_t np, uint64_t *out)
 {
     __shared__ unsigned long long sh[kScanNT / 64];
+    if (n_dev) {
+        n = *n_dev;
+        np = (n + kScanTile - 1) / kScanTile;
+        if (np == 0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+            return;
+        }
+        if (blockIdx.x >= np) return;  // block-uniform
+    }
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanIPT;
     unsigned long long v[kScanIPT], s = 0;
 #pragma unroll
@@ -141,7 +153,8 @@ __global__ __launch_bounds__(kScanNT) void k_scan_apply(const uint64_t *in, uint
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = parts[np];
 }
 
-void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *parts, hipStream_t s)
+void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *parts, hipStream_t s,
+                     const uint64_t *n_dev)
 {
     uint64_t np = (n + kScanTile - 1) / kScanTile;
     if (np == 0) {
@@ -149,10 +162,10 @@ void launch_scan_u64(const uint64_t *in, uint64_t *out, uint64_t n, uint64_t *pa
         launch_fill_zero(out, 8, s);
         return;
     }
-    hipLaunchKernelGGL(k_scan_tile_sums, dim3((uint32_t)np), dim3(kScanNT), 0, s, in, n, parts);
-    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanNT), 0, s, parts, np);
-    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)np), dim3(kScanNT), 0, s, in, n, (const uint64_t *)parts, np,
-                       out);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3((uint32_t)np), dim3(kScanNT), 0, s, in, n, n_dev, parts);
+    hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(kScanNT), 0, s, parts, np, n_dev);
+    hipLaunchKernelGGL(k_scan_apply, dim3((uint32_t)np), dim3(kScanNT), 0, s, in, n, n_dev, (const uint64_t *)parts,
+                       np, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -893,7 +906,8 @@ __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
     __shared__ unsigned long long s_lo, s_hi;
     const uint32_t ngroups = a.plan[0];
     const uint64_t nd = (uint64_t)ngroups * ix.ntiles;
-    const uint64_t W = a.K ? a.koff[a.K] : 0;
+    const uint64_t K = a.Kd ? *a.Kd : a.K;
+    const uint64_t W = K ? a.koff[K] : 0;
     const uint64_t ns = (W + a.chunk - 1) / a.chunk;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1u);
@@ -907,7 +921,7 @@ __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
                                  a.out, a.out_stride, a.out_words, a.use_col, lds);
         } else if (item < nd + ns) {
             const uint64_t c0 = (item - nd) * a.chunk, c1 = (c0 + a.chunk < W) ? c0 + a.chunk : W;
-            sparse_chunk<NT>(ix, a.keys, a.kq, a.koff, a.K, c0, c1, a.out, a.out_stride,
+            sparse_chunk<NT>(ix, a.keys, a.kq, a.koff, K, c0, c1, a.out, a.out_stride,
                              reinterpret_cast<unsigned long long *>(lds), &s_lo, &s_hi);
         } else {
             break;

@@ -106,12 +106,51 @@ def test_graph_recapture_on_changes(c1):
     ]
     for name, act in steps:
         act()
-        for n in (40, 17, 40):  # a smaller batch in between changes the shape
+        # a shape runs as launches the first time and from a graph when it
+        # comes again; a smaller batch in between changes the shape
+        for n in (40, 40, 40, 17, 17, 40):
             sub_qo = qo[: n + 1]
             sub_qk = qk[: int(sub_qo[-1])]
             print(f"recapture {name} n={n}", flush=True)
             res = ctx.query_batch(sub_qk, sub_qo, ab.AB_PATH_THRESHOLD)
             _verify(csr, sub_qk, sub_qo, res, f"{name} n={n}")
+    ctx.close()
+
+
+def test_graph_reuse_across_key_counts(c1):
+    """One graph serves batches of the same query count whose key counts round
+    to the same capacity, with keys at a different device address each time:
+    the keys are staged and the key count is read on the device."""
+    import torch
+
+    ix, csr = c1
+    ctx = ab.Context(ix)
+    ctx.set_batch_mode(ab.Context.BATCH_GRAPH)
+    ctx.set_batch_threshold(0.01)
+    nq = 24
+    nv = configs.C1.nvalues
+    ids = torch.full((nq * nv,), -1, dtype=torch.int32, device="cuda")
+    cnt = torch.full((nq,), -1, dtype=torch.int32, device="cuda")
+    rng = np.random.default_rng(11)
+    for rep in range(8):
+        sizes = rng.integers(1, 300, nq)
+        if rep == 5:
+            sizes[:] = 0  # no keys at all
+        if rep == 6:
+            sizes[0] = 5000  # a larger key count: another capacity
+        qo = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        qk = rng.integers(0, configs.C1.nkeys + 10, int(qo[-1])).astype(np.uint32)
+        pad = int(rng.integers(0, 64))  # the keys start at a new address
+        dk = torch.from_numpy(np.concatenate([np.zeros(pad, np.uint32), qk]).view(np.int32)).cuda()
+        print(f"reuse rep={rep} K={int(qo[-1])}", flush=True)
+        ctx.query_batch_device(dk[pad:] if qk.size else dk, qo, ab.AB_PATH_THRESHOLD, None, ids, nv, cnt,
+                               want_paths=False)
+        ctx.sync()
+        h = ids.cpu().numpy().view(np.uint32)
+        c = cnt.cpu().numpy().view(np.uint32)
+        for i in range(nq):
+            want = csr.union(qk[qo[i]:qo[i + 1]])[1]
+            assert c[i] == want.size and np.array_equal(h[i * nv:i * nv + want.size], want), f"rep {rep} q{i}"
     ctx.close()
 
 
