@@ -1025,14 +1025,18 @@ static ab_status upload_keys(ab_ctx *c, const uint32_t *h_keys, uint64_t n)
     return AB_OK;
 }
 
-// Host-side choice for one query of host keys.
-static int choose_host(ab_ctx *c, const uint32_t *keys, uint32_t n, int path, uint64_t *M)
+// Host-side choice for one query of host keys, priced as a single query
+// (batch = false) or as a share of a batch (batch = true: the batch model and
+// tau_batch, as on the device).
+static int choose_host(ab_ctx *c, const uint32_t *keys, uint32_t n, int path, uint64_t *M, bool batch = false)
 {
     float f[AB_NFEATURES];
     host_features(c->ix->seg8.data(), c->ix->nkeys, c->ix->nvalues, c->ix->R, keys, n, f, M);
     if (path == AB_PATH_DENSE || path == AB_PATH_SPARSE) return path;
-    if ((path == AB_PATH_LEARNED || path == AB_PATH_AUTO) && c->mlp[0].loaded) return mlp_decide_host(c, 0, f);
-    return (double)*M >= c->tau * (double)c->ix->R ? AB_PATH_DENSE : AB_PATH_SPARSE;
+    const int slot = batch ? 1 : 0;
+    if ((path == AB_PATH_LEARNED || path == AB_PATH_AUTO) && c->mlp[slot].loaded) return mlp_decide_host(c, slot, f);
+    const double tau = batch ? c->tau_batch : c->tau;
+    return (double)*M >= tau * (double)c->ix->R ? AB_PATH_DENSE : AB_PATH_SPARSE;
 }
 
 extern "C" ab_status ab_query(ab_ctx *c, const uint32_t *h_keys, uint32_t n, int path, uint32_t *d_bitmap,
@@ -1199,6 +1203,31 @@ static ab_status query_batch(const char *fn, ab_ctx *c, const uint32_t *d_keys, 
         a.h_id_off = h_id_off;
         a.id_off_aligned = true;
         for (uint32_t q = 0; q < nq; ++q) a.id_off_aligned = a.id_off_aligned && (h_id_off[q] % 4 == 0);
+    }
+    // batches of a few queries with few keys: the keys come back to the host
+    // in one small copy and the choices are made there, as for single
+    // queries; the host-dispatched launches then need no device features and
+    // no second round trip (faster than a dozen device-dispatch launches)
+    constexpr uint32_t kHostChoiceQueries = 4;
+    constexpr uint64_t kHostChoiceKeys = 1u << 16;
+    const uint64_t K = h_qoff[nq];
+    if (nq >= 1 && nq <= kHostChoiceQueries && K <= kHostChoiceKeys) {
+        if (c->kstage_pending) {
+            HIP_TRY(hipEventSynchronize(c->kstage_ev));
+            c->kstage_pending = false;
+        }
+        AB_TRY(c->hkstage.ensure(K * 4 + 16));
+        if (K) HIP_TRY(hipMemcpyAsync(c->hkstage.p, d_keys, K * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        const uint32_t *hk = (const uint32_t *)c->hkstage.p;
+        uint8_t sel[kHostChoiceQueries];
+        uint64_t M[kHostChoiceQueries];
+        for (uint32_t q = 0; q < nq; ++q)
+            sel[q] = (uint8_t)choose_host(c, hk + h_qoff[q], (uint32_t)(h_qoff[q + 1] - h_qoff[q]), path, &M[q],
+                                          nq > 1);
+        a.h_sel = sel;
+        a.h_M = M;
+        return run_batch(c, a);
     }
     // the fused path's grids put queries on the y dimension (at most 65535)
     // and its persistent kernel is built for 256 and 512 threads
