@@ -274,7 +274,9 @@ ab_status ab_list_union_sparse_batch(ab_ctx *ctx, const uint32_t *h_keys,
  *   h_paths    nq bytes: the path each query took (AB_PATH_DENSE/SPARSE).
  * Queries of either path are packed into one union launch of the persistent
  * kernel; the whole batch is replayed from a cached HIP graph when the same
- * shape recurs. */
+ * shape recurs.  Batches of up to 4 queries with at most 65536 keys choose
+ * their paths on the host instead: their keys are copied back and the call
+ * waits for that copy. */
 ab_status ab_query_batch(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_qoff,
                          uint32_t nq, int path, uint32_t *d_bitmaps, uint32_t *d_ids,
                          uint64_t ids_stride, uint32_t *d_counts, uint8_t *h_paths);
