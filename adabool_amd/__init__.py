@@ -405,6 +405,12 @@ class Index:
         _check(_lib.ab_query_id_bound(self._h, _ptr(k), k.size, C.byref(b)))
         return b.value
 
+    def key_segments(self, t: int) -> np.ndarray:
+        """key_segments_ prefix of record type t (nkeys + 1 u32)."""
+        segs = np.zeros(self.info["nkeys"] + 1, np.uint32)
+        _check(_lib.ab_index_export_type(self._h, t, None, None, _ptr(segs)))
+        return segs
+
     def export_type(self, t: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
         """(headers, payload bytes, key_segments prefix) of record type t."""
         inf = self.info

@@ -51,15 +51,9 @@ _SEG_CACHE = {}
 
 
 def _segs(ix, t):
-    import ctypes as C
-
-    import adabool_amd as ab
-
     key = (id(ix), t)
     if key not in _SEG_CACHE:
-        segs = np.zeros(ix.info["nkeys"] + 1, np.uint32)
-        ab._check(ab._lib.ab_index_export_type(ix._h, t, None, None, segs.ctypes.data))
-        _SEG_CACHE[key] = segs
+        _SEG_CACHE[key] = ix.key_segments(t)
     return _SEG_CACHE[key]
 
 

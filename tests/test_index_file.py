@@ -1,7 +1,6 @@
 """Binary index file: save -> load reproduces every array of both layouts
 (checked through the exports and the full tile decode), and damaged files
 are refused.  Host-only handles: runs without a GPU."""
-import os
 
 import numpy as np
 import pytest
