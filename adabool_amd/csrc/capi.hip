@@ -997,7 +997,7 @@ ab_status ab::run_batch(ab_ctx *c, const BatchArgs &a)
             launch_popcount(bm, out_stride, out_words, nq, (uint32_t *)c->ccnt.p, s);
         }
         TimedSpan ts(c, AB_K_WRITE_IDS);
-        launch_write_ids(bm, out_stride, out_words, nq, (const uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, d_idoff,
+        launch_write_ids(bm, out_stride, out_words, nq, (uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, d_idoff,
                          a.id_off_aligned, a.d_counts, s);
     }
     HIP_TRY(hipGetLastError());
@@ -1266,7 +1266,7 @@ extern "C" ab_status ab_compact_ids(ab_ctx *c, const uint32_t *d_bitmaps, uint32
     const uint64_t nchunk = (w + kCompactWords - 1) / kCompactWords;
     AB_TRY(c->ccnt.ensure((size_t)nq * nchunk * 4 + 4));
     launch_popcount(d_bitmaps, w, w, nq, (uint32_t *)c->ccnt.p, c->stream);
-    launch_write_ids(d_bitmaps, w, w, nq, (const uint32_t *)c->ccnt.p, d_ids, ids_stride, nullptr, false, d_counts,
+    launch_write_ids(d_bitmaps, w, w, nq, (uint32_t *)c->ccnt.p, d_ids, ids_stride, nullptr, false, d_counts,
                      c->stream);
     HIP_TRY(hipGetLastError());
     return AB_OK;

@@ -144,7 +144,7 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
         }
         TimedSpan ts(c, AB_K_WRITE_IDS);
         const uint64_t *d_idoff = p.packed ? qoff + (nq + 1) : nullptr;
-        launch_write_ids(p.bm, p.stride, p.words, nq, (const uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, d_idoff,
+        launch_write_ids(p.bm, p.stride, p.words, nq, (uint32_t *)c->ccnt.p, a.d_ids, a.ids_stride, d_idoff,
                          a.id_off_aligned, a.d_counts, s);
     }
     HIP_TRY(hipGetLastError());

@@ -102,8 +102,10 @@ void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32
                      uint32_t *ccnt, hipStream_t s);
 // Query q's ids go to ids + id_off[q] (device offsets) or, with id_off NULL,
 // to ids + q * ids_stride; off_aligned: every id_off[q] is a multiple of 4.
+// ccnt (from launch_popcount) is turned into per-query exclusive prefixes in
+// place (counts[q] = the totals, when counts is not NULL); ids NULL: counts only.
 void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
-                      const uint32_t *ccnt, uint32_t *ids, uint64_t ids_stride, const uint64_t *id_off,
+                      uint32_t *ccnt, uint32_t *ids, uint64_t ids_stride, const uint64_t *id_off,
                       bool off_aligned, uint32_t *counts, hipStream_t s);
 void launch_features(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff, uint32_t nq,
                      uint64_t R, float *feats, uint64_t *M, hipStream_t s);

@@ -1018,22 +1018,35 @@ void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32
 // up; the wave then stores whole quads with 128-bit writes (ALIGNED: ids +
 // q * ids_stride is 16-byte aligned) and the partial end quads dword by dword.
 constexpr uint32_t kStage = 64 * 32 + 4;
+// Per query, in place: chunk counts -> exclusive prefix; counts[q] = total.
+__global__ __launch_bounds__(kScanNT) void k_chunk_scan(uint32_t *ccnt, uint32_t nchunk, uint32_t *counts)
+{
+    __shared__ unsigned long long sh[kScanNT / 64];
+    uint32_t *cc = ccnt + (uint64_t)blockIdx.x * nchunk;
+    unsigned long long carry = 0;
+    for (uint32_t base = 0; base < nchunk; base += kScanNT) {
+        const uint32_t j = base + threadIdx.x;
+        const unsigned long long x = j < nchunk ? cc[j] : 0;
+        unsigned long long tot;
+        const unsigned long long e = block_excl_scan(x, sh, &tot);
+        if (j < nchunk) cc[j] = (uint32_t)(carry + e);
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && counts) counts[blockIdx.x] = (uint32_t)carry;
+}
+
+// ccnt: per-chunk exclusive prefixes (k_chunk_scan).
 template <bool ALIGNED>
 __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
                                                    uint32_t nchunk, const uint32_t *__restrict__ ccnt,
                                                    uint32_t *__restrict__ ids, uint64_t ids_stride,
-                                                   const uint64_t *__restrict__ id_off, uint32_t *__restrict__ counts)
+                                                   const uint64_t *__restrict__ id_off)
 {
     __shared__ __attribute__((aligned(16))) uint32_t stage[4][kStage];
     __shared__ uint32_t wcnt[4];
-    __shared__ uint32_t sred[4];
     const uint32_t q = blockIdx.y, c = blockIdx.x;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    const uint32_t *cc = ccnt + (uint64_t)q * nchunk;
-    uint32_t pre = 0;
-    for (uint32_t i = threadIdx.x; i < c; i += 256) pre += cc[i];
-    pre = wave_sum(pre);
-    if (lane == 0) sred[wave] = pre;
+    const uint64_t prefix = ccnt[(uint64_t)q * nchunk + c];  // ids of the query's earlier chunks
     const uint32_t *src = bm + (uint64_t)q * stride;
     const uint64_t ww0 = (uint64_t)c * kCompactWords + (uint64_t)wave * 512;
     uint32_t x[8], mine = 0;
@@ -1046,9 +1059,6 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
     mine = wave_sum(mine);
     if (lane == 0) wcnt[wave] = mine;
     __syncthreads();
-    const uint64_t prefix = (uint64_t)sred[0] + sred[1] + sred[2] + sred[3];
-    if (c == nchunk - 1 && threadIdx.x == 0 && counts) counts[q] = (uint32_t)(prefix + cc[c]);
-    if (!ids) return;
     uint64_t off = prefix;
     for (uint32_t w = 0; w < wave; ++w) off += wcnt[w];
     uint32_t *dst = ids + (id_off ? id_off[q] : (uint64_t)q * ids_stride);
@@ -1095,19 +1105,21 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
     }
 }
 
-void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, const uint32_t *ccnt,
+void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, uint32_t *ccnt,
                       uint32_t *ids, uint64_t ids_stride, const uint64_t *id_off, bool off_aligned,
                       uint32_t *counts, hipStream_t s)
 {
     if (!nq || !words) return;
     uint32_t nchunk = (uint32_t)((words + kCompactWords - 1) / kCompactWords);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(nq), dim3(kScanNT), 0, s, ccnt, nchunk, counts);
+    if (!ids) return;
     const bool aligned = ((uintptr_t)ids % 16 == 0) && (id_off ? off_aligned : ids_stride % 4 == 0);
     if (aligned)
         hipLaunchKernelGGL(k_write_ids<true>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt, ids,
-                           ids_stride, id_off, counts);
+                           ids_stride, id_off);
     else
         hipLaunchKernelGGL(k_write_ids<false>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt,
-                           ids, ids_stride, id_off, counts);
+                           ids, ids_stride, id_off);
 }
 
 // ---------------------------------------------------------------------------

@@ -304,3 +304,62 @@ def test_c2_full_size_properties():
             assert counts[i] == want_ids.size
             if i < 6:
                 assert np.array_equal(ids[i], want_ids)
+
+
+def test_bitmap_paths_at_the_top_of_u32():
+    """nvalues = 2^32 - 1 (a 512 MB bitmap per query): the dense and sparse
+    single-query paths and a device-dispatched batch return the oracle's ids
+    for values next to 2^32 - 2, and the bitmap's last words carry them."""
+    import torch
+
+    nvalues = 2**32 - 1
+    top = nvalues - 1
+    rng = np.random.default_rng(0xADAB0801)
+    nk, n = 64, 6000
+    k = rng.integers(0, nk, n).astype(np.uint32)
+    v = (top - rng.integers(0, 70000, n)).astype(np.uint32)
+    k[:300] = 3  # one key with whole windows near the top -> bitsets
+    v[:300] = top - np.arange(300)
+    v[300:310] = np.arange(10) * 256  # and a few values at the bottom
+    ix = ab.Index(k, v, nk, nvalues)
+    ctx = ab.Context(ix)
+    csr = oracle.CSR(k, v, nk, nvalues)
+    sw = ix.stride_words
+    assert sw == 2**27
+    bm = torch.empty(sw, dtype=torch.int32, device="cuda")
+    ids = torch.empty(n + 16, dtype=torch.int32, device="cuda")
+    cnt = torch.empty(1, dtype=torch.int32, device="cuda")
+    qs = [np.array([3], np.uint32), np.arange(nk, dtype=np.uint32), rng.choice(nk, 9, replace=False).astype(np.uint32)]
+    for qi, q in enumerate(qs):
+        want = csr.union_ids(q)
+        for path in (ab.AB_PATH_DENSE, ab.AB_PATH_SPARSE):
+            print(f"top of u32 q{qi} path {path}", flush=True)
+            ctx.query_device(q, path, bm, ids, cnt)
+            ctx.sync()
+            c = int(cnt.cpu().numpy().view(np.uint32)[0])
+            got = ids[:c].cpu().numpy().view(np.uint32)
+            assert np.array_equal(got, want), f"q{qi} path {path}"
+            tail = bm[sw - 4096:].cpu().numpy().view(np.uint32)  # values 2^32 - 131072 .. 2^32 - 1
+            hi = want[want >= 2**32 - 131072] - (2**32 - 131072)
+            expect = np.zeros(4096, np.uint32)
+            np.bitwise_or.at(expect, (hi >> 5).astype(np.int64), (np.uint32(1) << (hi & 31).astype(np.uint32)))
+            assert np.array_equal(tail, expect), f"q{qi} path {path}: bitmap tail"
+    # a batch of six on the device-dispatched path (ids packed at their bounds)
+    qb = qs + [np.array([1, 2], np.uint32), np.zeros(0, np.uint32), np.array([nk + 3], np.uint32)]
+    qk = np.concatenate(qb)
+    qo = np.concatenate([[0], np.cumsum([q.size for q in qb])]).astype(np.uint64)
+    ub = [ix.id_bound(q) for q in qb]
+    off = np.concatenate([[0], np.cumsum([(u + 3) // 4 * 4 for u in ub])[:-1]]).astype(np.uint64)
+    bids = torch.full((int(sum((u + 3) // 4 * 4 for u in ub)) + 8,), -1, dtype=torch.int32, device="cuda")
+    bcnt = torch.empty(len(qb), dtype=torch.int32, device="cuda")
+    dk = torch.from_numpy(qk.view(np.int32)).cuda()
+    print("top of u32 batch", flush=True)
+    ctx.query_batch_packed(dk, qo, ab.AB_PATH_THRESHOLD, bids, off, None, bcnt)
+    ctx.sync()
+    h = bids.cpu().numpy().view(np.uint32)
+    cc = bcnt.cpu().numpy().view(np.uint32)
+    for i, q in enumerate(qb):
+        want = csr.union_ids(q)
+        assert cc[i] == want.size and np.array_equal(h[int(off[i]):int(off[i]) + want.size], want), f"batch q{i}"
+    ctx.close()
+    ix.close()
