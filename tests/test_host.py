@@ -213,3 +213,24 @@ def test_stats_match_oracle():
         a, b = ab.stats(d), oracle.stats(d)
         for key in a:
             assert a[key] == pytest.approx(b[key], rel=1e-12)
+
+
+def test_id_bound_and_argument_errors_on_host_index():
+    """The id upper bound from the host copy of the key prefixes: the sum of
+    the in-range keys' list lengths (repeats counted), capped at nvalues; and
+    the id-only entry points refuse bad arguments before touching a device."""
+    rng = np.random.default_rng(21)
+    keys = rng.integers(0, 40, 3000).astype(np.uint32)
+    vals = rng.integers(0, 5000, 3000).astype(np.uint32)
+    ix = ab.Index(keys, vals, 40, 5000, host_only=True)
+    pairs = np.unique((keys.astype(np.uint64) << 32) | vals)
+    per_key = np.bincount((pairs >> 32).astype(np.int64), minlength=40)
+    for q in ([], [3], [3, 3, 7], [39, 40, 10**6], list(range(40))):
+        want = min(int(sum(per_key[k] for k in q if k < 40)), 5000)
+        assert ix.id_bound(np.array(q, np.uint32)) == want, q
+    with pytest.raises(ab.AdaBoolError):
+        ab._check(ab._lib.ab_query_batch_ids(None, None, None, 0, None, None, None))
+    with pytest.raises(ab.AdaBoolError):
+        ab._check(ab._lib.ab_ctx_set_id_budget(None, 1 << 20))
+    with pytest.raises(ab.AdaBoolError):
+        ab._check(ab._lib.ab_query_id_bound(None, None, 0, None))
