@@ -291,7 +291,7 @@ extern "C" ab_status ab_index_save(const ab_index *ix, const char *path)
     };
     FileHeader hd{};
     memcpy(hd.magic, kMagic, 8);
-    hd.version = 1;
+    hd.version = 2;
     hd.nkeys = h->nkeys;
     hd.nvalues = h->nvalues;
     hd.tile_shift = h->tile_shift;
@@ -338,7 +338,7 @@ extern "C" ab_status ab_index_load(const char *path, int host_only, ab_index **o
     };
     FileHeader hd{};
     get(&hd, sizeof(hd));
-    if (!ok || memcmp(hd.magic, kMagic, 8) != 0 || hd.version != 1 || hd.nsections != 1 + 4 * kTypes + 3) {
+    if (!ok || memcmp(hd.magic, kMagic, 8) != 0 || hd.version != 2 || hd.nsections != 1 + 4 * kTypes + 3) {
         fclose(f);
         set_error(std::string("ab_index_load: not an index file of this version: ") + path);
         return AB_IO_ERROR;
@@ -509,7 +509,7 @@ extern "C" int64_t ab_index_decode_tiles(const ab_index *ix, uint32_t *ok, uint3
         const uint32_t t = m.tw & 15, wl = (m.tw >> 4) & 3, nv = m.tw >> 8, w = 1u << wl;
         for (uint32_t r = 0; r < nv; ++r) {
             const uint32_t lane = r & 63, slot = r >> 6;
-            const uint8_t *p = &h->vkeys[m.byte_off + ((uint64_t)lane * kSlots + slot) * w];
+            const uint8_t *p = &h->vkeys[((uint64_t)m.off_kb << 10) + ((uint64_t)lane * kSlots + slot) * w];
             uint32_t d = 0;
             for (uint32_t q = 0; q < w; ++q) d |= (uint32_t)p[q] << (8 * q);
             const uint32_t key = m.start_key + d;

@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_block_keys(VtIn v, uint4 *meta, const u
     const uint32_t b = blockIdx.x;
     uint4 m = meta[b];
     if (threadIdx.x == 0) {
-        m.y = (uint32_t)boff[b];
+        m.y = (uint32_t)(boff[b] >> 10);  // in units of 1024 bytes
         meta[b] = m;
     }
     const uint32_t t = m.z & 15u, wl = (m.z >> 4) & 3u, nv = m.z >> 8, w = 1u << wl;
@@ -572,8 +572,8 @@ extern "C" ab_status ab_index_build_device(const uint32_t *d_keys, const uint32_
         HIP_TRY(hipStreamSynchronize(s));
         kbytes = l[0] + l[1];
     }
-    if (kbytes >= 0xffffffffull) {
-        set_error("ab_index_build_device: key stream exceeds 4 GiB");
+    if ((kbytes >> 10) >= 0xffffffffull) {
+        set_error("ab_index_build_device: key stream exceeds 4 TiB");
         return AB_UNSUPPORTED;
     }
     void *vk = nullptr;

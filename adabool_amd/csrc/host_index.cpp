@@ -307,8 +307,8 @@ ab_status build_host_index(const uint32_t *keys, const uint32_t *vals, uint64_t 
         tile_nblk[j + 1] += tile_nblk[j];
         tile_nbytes[j + 1] += tile_nbytes[j];
     }
-    if (tile_nbytes[ntiles] >= 0xffffffffull) {
-        set_error("ab_index_build: key stream exceeds 4 GiB");
+    if ((tile_nbytes[ntiles] >> 10) >= 0xffffffffull) {
+        set_error("ab_index_build: key stream exceeds 4 TiB");
         return AB_UNSUPPORTED;
     }
     ix.tile_blk.resize(ntiles + 1);
@@ -328,7 +328,7 @@ ab_status build_host_index(const uint32_t *keys, const uint32_t *vals, uint64_t 
                     uint32_t wl = w == 1 ? 0 : (w == 2 ? 1 : 2);
                     BlockMeta m;
                     m.start_key = k0;
-                    m.byte_off = (uint32_t)off;
+                    m.off_kb = (uint32_t)(off >> 10);
                     m.tw = (uint32_t)t | (wl << 4) | (nv << 8);
                     m.rec_base = b;
                     ix.vblk[bi++] = m;

@@ -29,7 +29,8 @@ constexpr int kMinNum[kTypes] = {0, 1, 5, 9, 17, 0};
 // Block descriptor of the value-tiled key stream.
 struct BlockMeta {
     uint32_t start_key;  // key of the block's first record
-    uint32_t byte_off;   // offset of the block's 1024*w delta bytes
+    uint32_t off_kb;     // offset of the block's 1024*w delta bytes, in units of
+                         // 1024 bytes (every block is a multiple of that long)
     uint32_t tw;         // type | log2(width) << 4 | nvalid << 8
     uint32_t rec_base;   // first record (index into the type's VT arrays)
 };

@@ -648,7 +648,7 @@ __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, co
     for (uint32_t b = b0 + wave; b < b1; b += NW) {
         const uint4 mt = ix.vblk[b];
         const uint32_t type = mt.z & 15u, wl = (mt.z >> 4) & 3u, nv = mt.z >> 8;
-        const uint8_t *kp = ix.vkeys + mt.y;
+        const uint8_t *kp = ix.vkeys + ((uint64_t)mt.y << 10);
         // decode the block's keys (record s*64 + lane in slot s) into the wave's staging area
         if (wl == 0) {
             const uint4 v = *reinterpret_cast<const uint4 *>(kp + lane * 16u);
