@@ -895,7 +895,7 @@ __global__ void k_set_u32(uint32_t *p, uint32_t v)
 }
 
 // Persistent: every work-group claims items from one counter -- first the
-// dense (group, tile) items of all groups, then the sparse chunks -- until
+// dense (group, tile) items, tile by tile, then the sparse chunks -- until
 // none are left.  Dense items come first: they are the long ones.
 template <int MODE, int NT>
 __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
@@ -915,7 +915,9 @@ __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
         const uint64_t item = s_item;
         __syncthreads();
         if (item < nd) {
-            const uint32_t g = (uint32_t)(item / ix.ntiles), tile = (uint32_t)(item - (uint64_t)g * ix.ntiles);
+            // tile-major: the groups of one tile run close together in time,
+            // so the tile's key stream and records are still cached for each
+            const uint32_t tile = (uint32_t)(item / ngroups), g = (uint32_t)(item - (uint64_t)tile * ngroups);
             dense_tile<MODE, NT>(ix, tile, reinterpret_cast<const M *>(a.masks) + (uint64_t)g * a.mask_stride,
                                  a.anybms + (uint64_t)g * a.any_stride, a.grp_n[g], a.grp_q + (uint64_t)g * kGroupSlots,
                                  a.out, a.out_stride, a.out_words, a.use_col, lds);
