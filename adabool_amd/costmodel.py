@@ -175,8 +175,8 @@ def _train(ctx: Context, slot: int, F, T, tr, epochs: int):
 
 def main(argv: List[str] = None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="gpurun_out/cost_model_c4.json")
-    ap.add_argument("--data-out", default="gpurun_out/cost_timings_c4.npz")
+    ap.add_argument("--out", default="cost_model_c4.json")
+    ap.add_argument("--data-out", default="cost_timings_c4.npz")
     ap.add_argument("--queries", type=int, default=2000, help="timed queries per index")
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--repeats", type=int, default=2)

@@ -156,8 +156,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only (the path shards by value window across GPUs; "
-              "see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: measures one GPU only (across GPUs the path would shard by value window: each GPU "
+              "holds the records of its value range and writes its slice of every result). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch

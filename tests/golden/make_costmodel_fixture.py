@@ -3,12 +3,12 @@ queries (C2-shaped index, 4M values, Zipf 1.1) that the committed cost model
 was never trained on: features, M, R and both label sets.
 
 Input: the timing file written by `python -m adabool_amd.costmodel`
-(--data-out, default gpurun_out/cost_timings_c4.npz)."""
+(--data-out, default cost_timings_c4.npz)."""
 import sys
 
 import numpy as np
 
-src = sys.argv[1] if len(sys.argv) > 1 else "gpurun_out/cost_timings_c4.npz"
+src = sys.argv[1] if len(sys.argv) > 1 else "cost_timings_c4.npz"
 out = sys.argv[2] if len(sys.argv) > 2 else "tests/golden/cost_heldout.npz"
 d = np.load(src)
 h = np.nonzero(d["heldout"])[0]
