@@ -376,7 +376,8 @@ extern "C" ab_status ab_index_load(const char *path, int host_only, ab_index **o
     }
     // structural checks the kernels rely on
     bool good = h.seg8.size() == ((size_t)h.nkeys + 1) * kSegStride && h.tile_blk.size() == (size_t)h.ntiles + 1 &&
-                h.vkeys.size() >= 64;
+                h.vkeys.size() >= 64 && h.tile_shift >= 8 && h.tile_shift <= 16 &&
+                (uint64_t)h.ntiles == (((uint64_t)h.nvalues + (1ull << h.tile_shift) - 1) >> h.tile_shift);
     for (int t = 0; t < kTypes && good; ++t)
         good = h.pay[t].size() == h.hdr[t].size() * kPayloadBytes[t] && h.vhdr[t].size() == h.hdr[t].size() &&
                h.vpay[t].size() == h.pay[t].size() && h.seg8[(size_t)h.nkeys * kSegStride + t] == h.hdr[t].size();
