@@ -4,8 +4,9 @@
 // standard deviation kept with the model.
 //
 // Kernels (all hand-written, FP32):
-//   k_mlp_decide  -- one thread per query, parameters staged in LDS; writes
-//                    the chosen path (and optionally the two predictions).
+//   k_mlp_decide  -- one wave per query, one lane per hidden unit, weights
+//                    staged transposed in LDS; writes the chosen path (and
+//                    optionally the two predictions).
 //   k_mlp_step    -- one work-group runs a whole minibatch step: forward with
 //                    activations in a global scratch, backward layer by layer
 //                    (dW = dZ^T A reductions split over the 256 threads), then
@@ -65,24 +66,70 @@ __device__ __forceinline__ void load_x(const float *__restrict__ X, uint32_t row
     for (int f = 0; f < kF; ++f) x[f] = (X[(uint64_t)row * kF + f] - norm[f]) / norm[kF + f];
 }
 
-__global__ __launch_bounds__(256) void k_mlp_decide(const float *__restrict__ params, const float *__restrict__ norm,
-                                                    const float *__restrict__ X, uint32_t n, uint8_t *sel,
-                                                    float *pred)
+// One wave per query, lane j computing hidden unit j of both layers (and
+// lanes 0..kO-1 the outputs).  Every unit accumulates its fmaf terms in the
+// same order as mlp_fwd1, so the predictions are bit-identical to it.  The
+// weight matrices are staged transposed (W^T[i][j] at i * rows + j) so the 64
+// lanes of a wave read 64 consecutive floats: no LDS bank conflicts.
+constexpr int kDecideQ = 4;  // queries (waves) per work-group
+static_assert(kH1 == 64 && kH2 == 64 && kO <= 64, "one lane per hidden unit");
+__global__ __launch_bounds__(64 * kDecideQ) void k_mlp_decide(const float *__restrict__ params,
+                                                             const float *__restrict__ norm,
+                                                             const float *__restrict__ X, uint32_t n, uint8_t *sel,
+                                                             float *pred)
 {
-    __shared__ float p[kParams];
+    __shared__ float w1t[kF * kH1];   // w1t[f * kH1 + j] = W1[j][f]
+    __shared__ float w2t[kH1 * kH2];  // w2t[i * kH2 + j] = W2[j][i]
     __shared__ float nm[2 * kF];
-    for (int i = threadIdx.x; i < kParams; i += 256) p[i] = params[i];
+    __shared__ float hb[kDecideQ][64];
+    for (int i = threadIdx.x; i < kF * kH1; i += 64 * kDecideQ) {
+        const int j = i / kF, f = i - j * kF;
+        w1t[f * kH1 + j] = params[oW1 + i];
+    }
+    for (int i = threadIdx.x; i < kH1 * kH2; i += 64 * kDecideQ) {
+        const int j = i / kH1, k = i - j * kH1;
+        w2t[k * kH2 + j] = params[oW2 + i];
+    }
     if (threadIdx.x < 2 * kF) nm[threadIdx.x] = norm[threadIdx.x];
     __syncthreads();
-    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
-    if (q >= n) return;
-    float x[kF], y[kO];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * kDecideQ + wave;
+    if (q >= n) return;  // no work-group barrier below this point
+    float x[kF];
     load_x(X, q, nm, x);
-    mlp_fwd1(p, x, nullptr, nullptr, y);
-    if (sel) sel[q] = y[0] < y[1] ? (uint8_t)AB_PATH_DENSE : (uint8_t)AB_PATH_SPARSE;
-    if (pred) {
-        pred[2ull * q] = y[0];
-        pred[2ull * q + 1] = y[1];
+    float *h = hb[wave];
+    {
+        float s = params[ob1 + lane];
+#pragma unroll
+        for (int f = 0; f < kF; ++f) s = fmaf(w1t[f * kH1 + lane], x[f], s);
+        h[lane] = relu(s);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float s = params[ob2 + lane];
+#pragma unroll 16
+    for (int i = 0; i < kH1; ++i) s = fmaf(w2t[i * kH2 + lane], h[i], s);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    h[lane] = relu(s);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float y = 0.f;
+    if (lane < (uint32_t)kO) {
+        y = params[ob3 + lane];
+#pragma unroll 16
+        for (int j = 0; j < kH2; ++j) y = fmaf(params[oW3 + lane * kH2 + j], h[j], y);
+    }
+    const float y0 = __shfl(y, 0, 64), y1 = __shfl(y, 1, 64);
+    if (lane == 0) {
+        if (sel) sel[q] = y0 < y1 ? (uint8_t)AB_PATH_DENSE : (uint8_t)AB_PATH_SPARSE;
+        if (pred) {
+            pred[2ull * q] = y0;
+            pred[2ull * q + 1] = y1;
+        }
     }
 }
 
@@ -212,7 +259,7 @@ ab_status mlp_decide(ab_ctx *c, int slot, const float *d_feat, uint32_t nq, uint
         return AB_INVALID_ARG;
     }
     if (!nq) return AB_OK;
-    hipLaunchKernelGGL(k_mlp_decide, dim3((nq + 255) / 256), dim3(256), 0, c->stream, (const float *)md.params.p,
+    hipLaunchKernelGGL(k_mlp_decide, dim3((nq + kDecideQ - 1) / kDecideQ), dim3(64 * kDecideQ), 0, c->stream, (const float *)md.params.p,
                        (const float *)md.norm.p, d_feat, nq, d_sel, (float *)nullptr);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_DEVICE_ERROR;
 }
@@ -336,7 +383,7 @@ extern "C" ab_status ab_mlp_predict(ab_ctx *c, int slot, const float *h_x, uint3
     ab_status st = AB_OK;
     if (hipMemcpyAsync(x.p, h_x, (size_t)n * kF * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess) st = AB_DEVICE_ERROR;
     if (st == AB_OK) {
-        hipLaunchKernelGGL(k_mlp_decide, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const float *)md.params.p,
+        hipLaunchKernelGGL(k_mlp_decide, dim3((n + kDecideQ - 1) / kDecideQ), dim3(64 * kDecideQ), 0, c->stream, (const float *)md.params.p,
                            (const float *)md.norm.p, (const float *)x.p, n, (uint8_t *)s.p, (float *)y.p);
         if (hipMemcpyAsync(h_y, y.p, (size_t)n * kO * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             (h_sel && hipMemcpyAsync(h_sel, s.p, n, hipMemcpyDeviceToHost, c->stream) != hipSuccess) ||
