@@ -61,7 +61,10 @@ void launch_zero_qlist(uint32_t *bm, uint64_t stride, uint64_t words, const uint
 {
     if (!n || !words) return;
     uint32_t gx = (uint32_t)((words + kZeroChunk - 1) / kZeroChunk);
-    hipLaunchKernelGGL(k_zero_qlist, dim3(gx, n), dim3(256), 0, s, bm, stride, words, qlist);
+    for (uint32_t i0 = 0; i0 < n; i0 += 65535) {  // the list goes on the grid's y dimension
+        const uint32_t m = n - i0 < 65535 ? n - i0 : 65535;
+        hipLaunchKernelGGL(k_zero_qlist, dim3(gx, m), dim3(256), 0, s, bm, stride, words, qlist + i0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1006,11 +1009,18 @@ __global__ __launch_bounds__(256) void k_popcount(const uint32_t *__restrict__ b
     if (threadIdx.x == 0) ccnt[(uint64_t)q * nchunk + c] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// queries go on the grid's y dimension, in slices of at most 65535
+constexpr uint32_t kMaxGridY = 65535;
+
 void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, uint32_t *ccnt, hipStream_t s)
 {
     if (!nq || !words) return;
     uint32_t nchunk = (uint32_t)((words + kCompactWords - 1) / kCompactWords);
-    hipLaunchKernelGGL(k_popcount, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt);
+    for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
+        const uint32_t n = nq - q0 < kMaxGridY ? nq - q0 : kMaxGridY;
+        hipLaunchKernelGGL(k_popcount, dim3(nchunk, n), dim3(256), 0, s, bm + (uint64_t)q0 * stride, stride, words,
+                           nchunk, ccnt + (uint64_t)q0 * nchunk);
+    }
 }
 
 // A work-group expands one chunk of 2048 words in 8 steps of 256 words; in
@@ -1124,12 +1134,19 @@ void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint3
     hipLaunchKernelGGL(k_chunk_scan, dim3(nq), dim3(kScanNT), 0, s, ccnt, nchunk, counts);
     if (!ids) return;
     const bool aligned = ((uintptr_t)ids % 16 == 0) && (id_off ? off_aligned : ids_stride % 4 == 0);
-    if (aligned)
-        hipLaunchKernelGGL(k_write_ids<true>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt, ids,
-                           ids_stride, id_off);
-    else
-        hipLaunchKernelGGL(k_write_ids<false>, dim3(nchunk, nq), dim3(256), 0, s, bm, stride, words, nchunk, ccnt,
-                           ids, ids_stride, id_off);
+    for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
+        const uint32_t n = nq - q0 < kMaxGridY ? nq - q0 : kMaxGridY;
+        const uint32_t *b = bm + (uint64_t)q0 * stride;
+        const uint32_t *cc = ccnt + (uint64_t)q0 * nchunk;
+        uint32_t *dst = id_off ? ids : ids + (uint64_t)q0 * ids_stride;
+        const uint64_t *off = id_off ? id_off + q0 : nullptr;
+        if (aligned)
+            hipLaunchKernelGGL(k_write_ids<true>, dim3(nchunk, n), dim3(256), 0, s, b, stride, words, nchunk, cc, dst,
+                               ids_stride, off);
+        else
+            hipLaunchKernelGGL(k_write_ids<false>, dim3(nchunk, n), dim3(256), 0, s, b, stride, words, nchunk, cc,
+                               dst, ids_stride, off);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -78,6 +78,44 @@ def test_packed_batch(c1, mode, aligned):
     ctx.close()
 
 
+def test_batch_beyond_grid_limit(c1):
+    """70000 queries in one packed batch: more than a grid's y dimension holds
+    (65535), so the batch runs host-dispatched and its zero-fill, popcount and
+    id writer launch in slices of queries."""
+    import torch
+
+    ix, csr = c1
+    ctx = ab.Context(ix)
+    ctx.set_batch_threshold(0.01)
+    rng = np.random.default_rng(0xADAB0503)
+    nq = 70000
+    qs = [rng.choice(configs.C1.nkeys, size=int(rng.integers(1, 3)), replace=False).astype(np.uint32)
+          for _ in range(nq)]
+    for i in range(0, nq, 5000):  # some large queries, which go dense
+        qs[i] = rng.choice(configs.C1.nkeys, size=300, replace=False).astype(np.uint32)
+    qs[nq - 1] = np.zeros(0, np.uint32)
+    qk, qo = _flat(qs)
+    want = [csr.union_ids(q) for q in qs]
+    counts = np.array([w.size for w in want], np.int64)
+    off = np.concatenate([[0], np.cumsum((counts + 3) // 4 * 4 + 4)[:-1]]).astype(np.uint64)
+    pos = int(off[-1] + (counts[-1] + 3) // 4 * 4 + 4)
+    dk = torch.from_numpy(qk.view(np.int32)).cuda()
+    ids = torch.full((pos + 8,), -1, dtype=torch.int32, device="cuda")
+    cnt = torch.empty(nq, dtype=torch.int32, device="cuda")
+    print(f"packed batch of {nq}", flush=True)
+    paths = ctx.query_batch_packed(dk, qo, ab.AB_PATH_THRESHOLD, ids, off, None, cnt)
+    ctx.sync()
+    assert (paths == ab.AB_PATH_DENSE).any() and (paths == ab.AB_PATH_SPARSE).any()
+    c = cnt.cpu().numpy().view(np.uint32)
+    assert np.array_equal(c.astype(np.int64), counts)
+    h = ids.cpu().numpy().view(np.uint32)
+    for i in list(range(0, nq, 97)) + list(range(0, nq, 5000)) + [65534, 65535, 65536, nq - 2, nq - 1]:
+        a, n = int(off[i]), int(counts[i])
+        assert np.array_equal(h[a:a + n], want[i]), f"q{i} ids"
+        assert (h[a + n:a + n + 4] == 0xFFFFFFFF).all(), f"q{i} wrote past its range"
+    ctx.close()
+
+
 def test_alone_equals_in_mix(c1):
     ix, csr = c1
     qs = _mix(0xADAB0502, 300)
