@@ -114,13 +114,38 @@ def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float
     n1 = int(min(nq, max(1, budget_s / 4 / (per_q * nthreads))))
     t1, _, lat1, _ = csr.union_batch(keys[: int(qoff[n1])], qoff[: n1 + 1], 1)
     what = f"{passes} passes over all {nq}" if passes > 1 else f"first {ns} of the {nq}"
+    coo = coo_scan_c1()
     return {"value": ns * passes / t, "unit": "queries/sec", "cores": nthreads, "kind": "port",
             "sample": f"{what} C2 queries, oracle/oracle.c or_union_batch, {nthreads} threads, "
                       f"{t:.1f} s", "counts_match_gpu": ok, "cpu_model": cpu_model(),
             "host_cpus_visible": os.cpu_count(),
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "single_thread": {"value": n1 / t1, "queries": n1, "p50_ms": float(np.percentile(lat1, 50)),
-                              "p99_ms": float(np.percentile(lat1, 99))}}
+                              "p99_ms": float(np.percentile(lat1, 99))},
+            "coo_scan_c1": coo}
+
+
+def coo_scan_c1(reps: int = 20):
+    """Sanity figure: the reference's own ground-truth method (a scan of every
+    pair per query, test/adsfilter_q_all.cu:196-208, restated as or_scan_coo)
+    on C1 (1M pairs, one query of 64 distinct uniform keys), one thread; its
+    bitmap is checked against the CSR union."""
+    import oracle
+    from adabool_amd import configs
+
+    k, v = configs.make_filter(configs.C1)
+    qk, qo = configs.make_queries(configs.C1, configs.C1_Q)
+    q = qk[qo[0]:qo[1]]
+    want = oracle.CSR(k, v, configs.C1.nkeys, configs.C1.nvalues).union(q)[0]
+    lat = []
+    ok = True
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        bm = oracle.scan_coo(k, v, q, configs.C1.nvalues)
+        lat.append(time.perf_counter() - t0)
+        ok = ok and bool(np.array_equal(bm, want))
+    return {"value": 1.0 / float(np.median(lat)), "unit": "queries/sec", "cores": 1, "pairs": int(k.size),
+            "queries": reps, "p50_ms": float(np.median(lat)) * 1e3, "matches_csr_union": ok}
 
 
 def main():
