@@ -166,7 +166,9 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
         p.Kcap = 4096;
         while (p.Kcap < p.K) p.Kcap <<= 1;
     }
-    p.G = std::max<uint32_t>(1, max_group(c));
+    // a group never needs to be wider than the batch: a narrower group takes
+    // narrower masks (u32 up to 32 queries) and less LDS
+    p.G = std::max<uint32_t>(1, std::min<uint32_t>(max_group(c), a.nq));
     p.mode = p.G <= 32 ? kModeMask32 : (p.G <= 64 ? kModeMask64 : kModeMask128);
     p.use_col = col_fits(c, p.mode, p.G);
     HIP_TRY(fused_grid(p.mode, c->dense_threads, ix->dev, p.G, p.use_col, &p.grid));
