@@ -542,8 +542,9 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
 
 // One 1024-record block of type T whose absolute keys are staged in kb
 // (record s*64 + lane at kb[s*64 + lane]).  Slots go RS at a time (eight for
-// the narrow types, whose round fits in few registers; four for the wide
-// ones): the RS any-bit probes first; then, for every slot whose key some
+// ARRAY1, whose round fits in few registers; four for the others, which
+// measured 1% faster than eight for ARRAY4/8 once every record used the
+// column masks): the RS any-bit probes first; then, for every slot whose key some
 // query of the group holds, the query-mask load and the header/payload loads
 // together (the any-bit is exactly "mask != 0"); then the RS emits -- unrolled
 // for ARRAY1, whose emit is a few instructions, else taken from the front of a
@@ -555,7 +556,7 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
                                             typename MaskOf<MODE>::T *col, uint32_t vmask)
 {
     using M = typename MaskOf<MODE>::T;
-    constexpr int RS = (T == AB_ARRAY1 || T == AB_ARRAY4 || T == AB_ARRAY8) ? 8 : 4;
+    constexpr int RS = (T == AB_ARRAY1) ? 8 : 4;
     constexpr int PW = TypeTraits<T>::P / 4 > 0 ? TypeTraits<T>::P / 4 : 1;  // payload words kept per slot
     const uint32_t *__restrict__ hdr = ix.vhdr[T];
     const uint8_t *__restrict__ pay = ix.vpay[T];
