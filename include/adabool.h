@@ -162,13 +162,14 @@ double ab_ctx_get_threshold(const ab_ctx *ctx);
 ab_status ab_ctx_set_batch_threshold(ab_ctx *ctx, double tau);
 double ab_ctx_get_batch_threshold(const ab_ctx *ctx);
 /* Dense group width (queries per key-stream pass, 1..128; host-dispatched
- * batches, mode 0, use at most 64). */
+ * batches, mode 0, use at most 64; a device-dispatched batch of fewer
+ * queries uses its own query count as the width). */
 ab_status ab_ctx_set_dense_group(ab_ctx *ctx, uint32_t width);
 /* Work-group size of the dense kernel (64, 128, 256 or 512 threads; default 512). */
 ab_status ab_ctx_set_dense_threads(ab_ctx *ctx, uint32_t threads);
 /* Per-value query-set accumulation in dense groups (on by default; used when
- * it fits the work-group's LDS).  Off: every match is OR-ed into each
- * matching query's row tile. */
+ * it fits the work-group's LDS, and then by every matched record).  Off: every
+ * match is OR-ed into each matching query's row tile. */
 ab_status ab_ctx_set_dense_columns(ab_ctx *ctx, int on);
 /* Records per work item of the sparse kernel (power of two, 256..8192). */
 ab_status ab_ctx_set_sparse_chunk(ab_ctx *ctx, uint32_t items);
