@@ -73,7 +73,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default="tuning_c2.json")
     ap.add_argument("--batches", default="1,4,16,64,256,1024,4096")
-    ap.add_argument("--tile-shifts", default="10,11,12")
+    ap.add_argument("--tile-shifts", default="9,10,11,12")
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     a = ap.parse_args(argv)
     spec = configs.C2

@@ -675,7 +675,7 @@ extern "C" double ab_ctx_get_batch_threshold(const ab_ctx *c) { return c ? c->ta
 extern "C" ab_status ab_ctx_set_dense_group(ab_ctx *c, uint32_t w)
 {
     if (!c || w < 1 || w > kGroupSlots) {
-        set_error("ab_ctx_set_dense_group: width must be 1..128");
+        set_error("ab_ctx_set_dense_group: width must be 1..256");
         return AB_INVALID_ARG;
     }
     c->group = w;
@@ -800,7 +800,7 @@ extern "C" ab_status ab_ctx_timing(ab_ctx *c, double *ms, uint64_t *n, int reset
 // ---------------------------------------------------------------------------
 int ab::group_mode(uint32_t B)
 {
-    return B == 1 ? kModeBitmap : (B <= 32 ? kModeMask32 : (B <= 64 ? kModeMask64 : kModeMask128));
+    return B == 1 ? kModeBitmap : (B <= 32 ? kModeMask32 : (B <= 64 ? kModeMask64 : (B <= 128 ? kModeMask128 : kModeMask256)));
 }
 
 constexpr uint32_t kLdsBudget = 65536;

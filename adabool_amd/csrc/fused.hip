@@ -169,7 +169,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     // a group never needs to be wider than the batch: a narrower group takes
     // narrower masks (u32 up to 32 queries) and less LDS
     p.G = std::max<uint32_t>(1, std::min<uint32_t>(max_group(c), a.nq));
-    p.mode = p.G <= 32 ? kModeMask32 : (p.G <= 64 ? kModeMask64 : kModeMask128);
+    p.mode = p.G <= 32 ? kModeMask32 : (p.G <= 64 ? kModeMask64 : (p.G <= 128 ? kModeMask128 : kModeMask256));
     p.use_col = col_fits(c, p.mode, p.G);
     HIP_TRY(fused_grid(p.mode, c->dense_threads, ix->dev, p.G, p.use_col, &p.grid));
     c->fused_grid = p.grid;
@@ -180,7 +180,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     const uint64_t nkeys = ix->nkeys;
     p.mask_stride = (nkeys + 63) / 64 * 64;
     p.any_stride = (nkeys + 31) / 32 + 16;
-    const size_t esize = p.mode == kModeMask32 ? 4 : (p.mode == kModeMask64 ? 8 : 16);
+    const size_t esize = p.mode == kModeMask32 ? 4 : (p.mode == kModeMask64 ? 8 : (p.mode == kModeMask128 ? 16 : 32));
 
     // ---- every buffer exists before anything is enqueued (capture-safe) ----
     const uint64_t nchunk = (p.words + kCompactWords - 1) / kCompactWords;

@@ -29,10 +29,10 @@ struct DenseGroup {
 };
 
 // mode of the dense kernel's key test
-enum DenseMode { kModeBitmap = 0, kModeMask32 = 1, kModeMask64 = 2, kModeMask128 = 3 };
+enum DenseMode { kModeBitmap = 0, kModeMask32 = 1, kModeMask64 = 2, kModeMask128 = 3, kModeMask256 = 4 };
 // query slots per dense group on the device-dispatched path (groups of up to
 // 128 queries use two 64-bit mask words)
-constexpr uint32_t kGroupSlots = 128;
+constexpr uint32_t kGroupSlots = 256;
 
 // scan scratch: parts needed for n items
 uint64_t scan_parts_needed(uint64_t n);

@@ -379,10 +379,15 @@ struct M128 {
 template <int MODE> struct MaskOf { using T = uint32_t; };
 template <> struct MaskOf<kModeMask64> { using T = unsigned long long; };
 template <> struct MaskOf<kModeMask128> { using T = M128; };
+struct M256 {
+    unsigned long long w[4];
+};
+template <> struct MaskOf<kModeMask256> { using T = M256; };
 
 __device__ __forceinline__ bool any_bits(uint32_t m) { return m != 0; }
 __device__ __forceinline__ bool any_bits(unsigned long long m) { return m != 0; }
 __device__ __forceinline__ bool any_bits(const M128 &m) { return (m.lo | m.hi) != 0; }
+__device__ __forceinline__ bool any_bits(const M256 &m) { return (m.w[0] | m.w[1] | m.w[2] | m.w[3]) != 0; }
 __device__ __forceinline__ uint32_t popc_mask(uint32_t m) { return (uint32_t)__popc(m); }
 __device__ __forceinline__ uint32_t popc_mask(unsigned long long m) { return (uint32_t)__popcll(m); }
 __device__ __forceinline__ uint32_t popc_mask(const M128 &m) { return (uint32_t)(__popcll(m.lo) + __popcll(m.hi)); }
@@ -400,6 +405,13 @@ template <class F> __device__ __forceinline__ void for_bits(const M128 &m, F &&f
     for_bits(m.lo, 0, f);
     for_bits(m.hi, 64, f);
 }
+template <class F> __device__ __forceinline__ void for_bits(const M256 &m, F &&f)
+{
+    for_bits(m.w[0], 0, f);
+    for_bits(m.w[1], 64, f);
+    for_bits(m.w[2], 128, f);
+    for_bits(m.w[3], 192, f);
+}
 __device__ __forceinline__ void or_mask(uint32_t *p, uint32_t m) { atomicOr(p, m); }
 __device__ __forceinline__ void or_mask(unsigned long long *p, unsigned long long m) { atomicOr(p, m); }
 // Both halves unconditionally: lanes of a wave hold different masks, and a
@@ -409,6 +421,11 @@ __device__ __forceinline__ void or_mask(M128 *p, const M128 &m)
 {
     atomicOr(&p->lo, m.lo);
     atomicOr(&p->hi, m.hi);
+}
+__device__ __forceinline__ void or_mask(M256 *p, const M256 &m)
+{
+#pragma unroll
+    for (int i = 0; i < 4; ++i) atomicOr(&p->w[i], m.w[i]);
 }
 // 64 x 64 bit transpose across a wave: lane l holds row l (bit c = M[l][c]);
 // afterwards lane b holds column b (bit l = M[l][b]).  Six butterfly stages,
@@ -429,6 +446,10 @@ __device__ __forceinline__ unsigned long long transpose64(unsigned long long x, 
 __device__ __forceinline__ unsigned long long mask_word(uint32_t m, uint32_t) { return m; }
 __device__ __forceinline__ unsigned long long mask_word(unsigned long long m, uint32_t) { return m; }
 __device__ __forceinline__ unsigned long long mask_word(const M128 &m, uint32_t c) { return c ? m.hi : m.lo; }
+__device__ __forceinline__ unsigned long long mask_word(const M256 &m, uint32_t c)
+{
+    return c == 0 ? m.w[0] : (c == 1 ? m.w[1] : (c == 2 ? m.w[2] : m.w[3]));
+}
 // bit b of a mask in memory (global atomics)
 __device__ __forceinline__ void set_mask_bit(uint32_t *p, uint32_t b) { atomicOr(p, 1u << b); }
 __device__ __forceinline__ void set_mask_bit(unsigned long long *p, uint32_t b) { atomicOr(p, 1ull << b); }
@@ -436,6 +457,7 @@ __device__ __forceinline__ void set_mask_bit(M128 *p, uint32_t b)
 {
     atomicOr(b < 64 ? &p->lo : &p->hi, 1ull << (b & 63));
 }
+__device__ __forceinline__ void set_mask_bit(M256 *p, uint32_t b) { atomicOr(&p->w[b >> 6], 1ull << (b & 63)); }
 
 // Query bits of key k: the 1-bit-per-key bitmap answers alone for one query
 // and filters the per-key mask loads (8 bytes per key) for a group.
@@ -745,7 +767,7 @@ uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t thr
 {
     const uint32_t col = (mode == kModeBitmap || !use_col)
                              ? 0u
-                             : (1u << tile_shift) * (mode == kModeMask32 ? 4u : (mode == kModeMask64 ? 8u : 16u));
+                             : (1u << tile_shift) * (mode == kModeMask32 ? 4u : (mode == kModeMask64 ? 8u : (mode == kModeMask128 ? 16u : 32u)));
     return (threads / 64) * kBlockRecs * 4u + B * (1u << (tile_shift - 5)) * 4u + col;
 }
 
@@ -888,8 +910,11 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
     else if (mode == kModeMask64)
         hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks, mask_stride,
                            anybms, any_stride, nkeys, clear);
-    else
+    else if (mode == kModeMask128)
         hipLaunchKernelGGL(k_mask_groups<kModeMask128>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks,
+                           mask_stride, anybms, any_stride, nkeys, clear);
+    else
+        hipLaunchKernelGGL(k_mask_groups<kModeMask256>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks,
                            mask_stride, anybms, any_stride, nkeys, clear);
 }
 
@@ -960,8 +985,10 @@ hipError_t fused_grid(int mode, uint32_t threads, const DevIndex &ix, uint32_t G
         e = threads == 256 ? occupancy_nt<kModeMask32, 256>(lds, &per) : occupancy_nt<kModeMask32, 512>(lds, &per);
     else if (mode == kModeMask64)
         e = threads == 256 ? occupancy_nt<kModeMask64, 256>(lds, &per) : occupancy_nt<kModeMask64, 512>(lds, &per);
-    else
+    else if (mode == kModeMask128)
         e = threads == 256 ? occupancy_nt<kModeMask128, 256>(lds, &per) : occupancy_nt<kModeMask128, 512>(lds, &per);
+    else
+        e = threads == 256 ? occupancy_nt<kModeMask256, 256>(lds, &per) : occupancy_nt<kModeMask256, 512>(lds, &per);
     if (e != hipSuccess) return e;
     *grid = (uint32_t)(cus * (per < 1 ? 1 : per));
     return hipSuccess;
@@ -982,11 +1009,16 @@ hipError_t launch_union_fused(int mode, uint32_t threads, const DevIndex &ix, co
             hipLaunchKernelGGL((k_union_fused<kModeMask64, 256>), dim3(grid), dim3(256), lds, s, ix, a);
         else
             hipLaunchKernelGGL((k_union_fused<kModeMask64, 512>), dim3(grid), dim3(512), lds, s, ix, a);
-    } else {
+    } else if (mode == kModeMask128) {
         if (threads == 256)
             hipLaunchKernelGGL((k_union_fused<kModeMask128, 256>), dim3(grid), dim3(256), lds, s, ix, a);
         else
             hipLaunchKernelGGL((k_union_fused<kModeMask128, 512>), dim3(grid), dim3(512), lds, s, ix, a);
+    } else {
+        if (threads == 256)
+            hipLaunchKernelGGL((k_union_fused<kModeMask256, 256>), dim3(grid), dim3(256), lds, s, ix, a);
+        else
+            hipLaunchKernelGGL((k_union_fused<kModeMask256, 512>), dim3(grid), dim3(512), lds, s, ix, a);
     }
     return hipGetLastError();
 }

@@ -3,7 +3,7 @@
 //
 // Knobs (all on the device-dispatched batch path):
 //   dense group width  queries that share one pass over the value-tiled key
-//                      stream (1..128); sets the LDS row-tile width (group x
+//                      stream (1..256); sets the LDS row-tile width (group x
 //                      tile bits) and the query-mask word (u32, u64 or two
 //                      u64 for more than 64 queries)
 //   dense threads      work-group size of the persistent kernel (256 / 512),
@@ -30,7 +30,7 @@ extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const ui
         set_error("ab_ctx_autotune: bad argument");
         return AB_INVALID_ARG;
     }
-    const uint32_t groups[] = {8, 16, 32, 48, 64, 96, 128};
+    const uint32_t groups[] = {8, 16, 32, 48, 64, 96, 128, 192, 256};
     const uint32_t threads[] = {256, 512};
     const uint32_t chunks[] = {1024, 4096};
     const int columns[] = {1, 0};

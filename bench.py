@@ -172,7 +172,7 @@ def main():
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--tile-shift", type=int, default=0, help="value tile of the built index (0: automatic)")
-    ap.add_argument("--tile-shifts", default="10,12",
+    ap.add_argument("--tile-shifts", default="9,10,12",
                     help="value-tile sizes (log2) the autotuner also tries, one device-built index each")
     ap.add_argument("--streams", type=int, default=2,
                     help="contexts (one stream, graph and output buffers each) that consecutive timed steps "

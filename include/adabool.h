@@ -161,7 +161,7 @@ double ab_ctx_get_threshold(const ab_ctx *ctx);
  * over the index serves a whole group of dense queries (default 0.001). */
 ab_status ab_ctx_set_batch_threshold(ab_ctx *ctx, double tau);
 double ab_ctx_get_batch_threshold(const ab_ctx *ctx);
-/* Dense group width (queries per key-stream pass, 1..128; host-dispatched
+/* Dense group width (queries per key-stream pass, 1..256; host-dispatched
  * batches, mode 0, use at most 64; a device-dispatched batch of fewer
  * queries uses its own query count as the width). */
 ab_status ab_ctx_set_dense_group(ab_ctx *ctx, uint32_t width);
@@ -185,7 +185,7 @@ ab_status ab_ctx_set_batch_mode(ab_ctx *ctx, int mode);
 
 /* Autotuner.  Times ab_query_batch on the given batch (device keys, host
  * offsets; union only, no compaction) for every candidate of the grid
- *   dense group width {8,16,32,48,64,96,128} x dense threads {256,512}
+ *   dense group width {8,16,32,48,64,96,128,192,256} x dense threads {256,512}
  *   x sparse chunk {1024,4096} x dense columns {on,off},
  * `repeats` calls each after one warm-up call, and leaves the context set to
  * the fastest.  out (may be NULL) receives up to cap candidates, fastest

@@ -199,7 +199,7 @@ def test_fused_edge_filter(ts):
     for mode in (ab.Context.BATCH_FUSED, ab.Context.BATCH_GRAPH):
         ctx = ab.Context(ix)
         ctx.set_batch_mode(mode)
-        for group in (1, 2, 32, 64, 100, 128):
+        for group in (1, 2, 32, 64, 100, 128, 256):
             ctx.set_dense_group(group)
             for path in (ab.AB_PATH_DENSE, ab.AB_PATH_SPARSE):
                 print(f"edge ts={ts} mode={mode} group={group} path={path}", flush=True)

@@ -141,7 +141,7 @@ def test_batch_mixed_paths(c1):
         assert counts[i] == want_ids.size
 
 
-@pytest.mark.parametrize("group", [1, 7, 32, 33, 64, 65, 100, 128])
+@pytest.mark.parametrize("group", [1, 7, 32, 33, 64, 65, 100, 128, 129, 200, 256])
 def test_batch_dense_groups(c1, group):
     _, _, ix, ctx, csr = c1
     qk, qo = ab.gen_queries(0xADAB0102 + group, configs.C1.nkeys, 300, size_dist=1, p1=40.0, p2=1.0, lo=1,
