@@ -437,7 +437,23 @@ __device__ __forceinline__ unsigned long long transpose64(unsigned long long x, 
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         const uint32_t s = 32u >> k;
-        const unsigned long long y = __shfl_xor(x, (int)s, 64);
+        unsigned long long y;
+        if (s >= 4) {
+            y = __shfl_xor(x, (int)s, 64);
+        } else {
+            // lane ^ 1 and lane ^ 2 stay inside a quad: DPP quad permutes
+            constexpr int kQuad1 = 0xB1, kQuad2 = 0x4E;  // [1,0,3,2], [2,3,0,1]
+            const int lo = (int)(uint32_t)x, hi = (int)(uint32_t)(x >> 32);
+            int ylo, yhi;
+            if (s == 1) {
+                ylo = __builtin_amdgcn_mov_dpp(lo, kQuad1, 0xF, 0xF, false);
+                yhi = __builtin_amdgcn_mov_dpp(hi, kQuad1, 0xF, 0xF, false);
+            } else {
+                ylo = __builtin_amdgcn_mov_dpp(lo, kQuad2, 0xF, 0xF, false);
+                yhi = __builtin_amdgcn_mov_dpp(hi, kQuad2, 0xF, 0xF, false);
+            }
+            y = ((unsigned long long)(uint32_t)yhi << 32) | (uint32_t)ylo;
+        }
         x = (lane & s) ? ((x & ~kLow[k]) | ((y & ~kLow[k]) >> s)) : ((x & kLow[k]) | ((y & kLow[k]) << s));
     }
     return x;
