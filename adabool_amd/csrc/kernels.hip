@@ -429,7 +429,9 @@ __device__ __forceinline__ void or_mask(M256 *p, const M256 &m)
 }
 // 64 x 64 bit transpose across a wave: lane l holds row l (bit c = M[l][c]);
 // afterwards lane b holds column b (bit l = M[l][b]).  Six butterfly stages,
-// each swapping the off-diagonal s x s blocks between lanes l and l ^ s.
+// each swapping the off-diagonal s x s blocks between lanes l and l ^ s; the
+// partner's word comes by DPP or ds_swizzle where the pattern allows, and by
+// ds_bpermute only for lane ^ 32.
 __device__ __forceinline__ unsigned long long transpose64(unsigned long long x, uint32_t lane)
 {
     constexpr unsigned long long kLow[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
@@ -438,8 +440,26 @@ __device__ __forceinline__ unsigned long long transpose64(unsigned long long x, 
     for (int k = 0; k < 6; ++k) {
         const uint32_t s = 32u >> k;
         unsigned long long y;
-        if (s >= 4) {
-            y = __shfl_xor(x, (int)s, 64);
+        if (s == 32) {
+            y = __shfl_xor(x, 32, 64);
+        } else if (s == 16 || s == 4) {
+            // xor inside 32 lanes: ds_swizzle in bit-mask mode (and 0x1f, xor s)
+            const int lo = (int)(uint32_t)x, hi = (int)(uint32_t)(x >> 32);
+            int ylo, yhi;
+            if (s == 16) {
+                ylo = __builtin_amdgcn_ds_swizzle(lo, 0x401F);
+                yhi = __builtin_amdgcn_ds_swizzle(hi, 0x401F);
+            } else {
+                ylo = __builtin_amdgcn_ds_swizzle(lo, 0x101F);
+                yhi = __builtin_amdgcn_ds_swizzle(hi, 0x101F);
+            }
+            y = ((unsigned long long)(uint32_t)yhi << 32) | (uint32_t)ylo;
+        } else if (s == 8) {
+            // lane ^ 8 is a rotation by 8 inside a row of 16: DPP row_ror:8
+            const int lo = (int)(uint32_t)x, hi = (int)(uint32_t)(x >> 32);
+            const int ylo = __builtin_amdgcn_mov_dpp(lo, 0x128, 0xF, 0xF, false);
+            const int yhi = __builtin_amdgcn_mov_dpp(hi, 0x128, 0xF, 0xF, false);
+            y = ((unsigned long long)(uint32_t)yhi << 32) | (uint32_t)ylo;
         } else {
             // lane ^ 1 and lane ^ 2 stay inside a quad: DPP quad permutes
             constexpr int kQuad1 = 0xB1, kQuad2 = 0x4E;  // [1,0,3,2], [2,3,0,1]
