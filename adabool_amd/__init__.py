@@ -41,7 +41,7 @@ __all__ = [
     "AB_PATH_AUTO", "AB_PATH_DENSE", "AB_PATH_SPARSE", "AB_PATH_THRESHOLD", "AB_PATH_LEARNED", "AB_PATH_IDS",
     "NTYPES", "PAYLOAD_BYTES", "NFEATURES", "AdaBoolError", "lib_path", "load_library",
     "Index", "Context", "IvfList", "gen_pairs", "gen_queries", "read_coo", "read_query",
-    "stats", "key_block_meta",
+    "stats", "key_block_meta", "find_query_files",
 ]
 
 AB_PATH_AUTO, AB_PATH_DENSE, AB_PATH_SPARSE, AB_PATH_THRESHOLD, AB_PATH_LEARNED, AB_PATH_IDS = range(6)
@@ -112,8 +112,8 @@ _SIGS = {
     "ab_ctx_set_timing": (C.c_int, [_VP, C.c_int]),
     "ab_ctx_timing": (C.c_int, [_VP, _VP, _VP, C.c_int]),
     "ab_query": (C.c_int, [_VP, _VP, C.c_uint32, C.c_int, _VP, _VP, _VP, C.POINTER(C.c_int)]),
-    "ab_list_union_sparse": (C.c_int, [_VP, _VP, C.c_uint32, _VP]),
-    "ab_list_union_dense": (C.c_int, [_VP, C.c_uint64, _VP, _VP]),
+    "ab_list_union_sparse": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint64]),
+    "ab_list_union_dense": (C.c_int, [_VP, C.c_uint64, _VP, _VP, C.c_uint64]),
     "ab_list_union_dense_batch": (C.c_int, [_VP, C.c_uint32, C.c_uint64, _VP, _VP, C.c_uint32]),
     "ab_list_union_sparse_batch": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_uint32, _VP, C.c_uint32]),
     "ab_query_batch": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, _VP, _VP, C.c_uint64, _VP, _VP]),
@@ -146,6 +146,7 @@ _SIGS = {
     "ab_dataset_free": (None, [_VP]),
     "ab_read_query": (C.c_int64, [C.c_char_p, _VP, C.c_uint64]),
     "ab_read_coo": (C.c_int64, [C.c_char_p, _VP, _VP, C.c_uint64, _U32P, _U32P]),
+    "ab_find_query_files": (C.c_int64, [C.c_char_p, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]),
     "ab_stats": (None, [_VP, C.c_uint64, _VP]),
 }
 
@@ -216,6 +217,19 @@ def _ptr(a) -> int:
     raise TypeError(f"cannot take the address of {type(a)}")
 
 
+def _buf_bytes(a, n: Optional[int]) -> int:
+    """Bytes a device buffer holds: n when given (never more than a tensor's
+    own size), else the tensor's size."""
+    if hasattr(a, "element_size"):
+        have = a.numel() * a.element_size()
+        if n is not None and n > have:
+            raise ValueError(f"res_len_bytes {n} exceeds the tensor's {have} bytes")
+        return have if n is None else n
+    if n is None:
+        raise ValueError("res_len_bytes is required for a raw device address")
+    return n
+
+
 def _u32(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
 
@@ -269,6 +283,23 @@ def read_query(path: str) -> np.ndarray:
     out = np.zeros(n, np.uint32)
     _lib.ab_read_query(path.encode(), _ptr(out), n)
     return out
+
+
+def find_query_files(dir_path: str) -> list:
+    """FindQueryFiles (test/adsfilter_q_all.cu:211-231): paths of the regular
+    files named query_vec_<digits>.coo in dir_path, sorted lexicographically.
+    As the reference does, an unreadable directory is reported on stderr and
+    gives an empty list."""
+    load_library()
+    need = C.c_uint64()
+    n = _lib.ab_find_query_files(dir_path.encode(), None, 0, C.byref(need))
+    if n < 0:
+        import sys
+        print(f"Error reading directory: {_lib.ab_last_error().decode()}", file=sys.stderr)
+        return []
+    buf = C.create_string_buffer(max(need.value, 1))
+    n = _lib.ab_find_query_files(dir_path.encode(), buf, need.value, C.byref(need))
+    return [x.decode() for x in buf.raw[:need.value].split(b"\0")[:n]]
 
 
 def read_coo(path: str) -> Tuple[np.ndarray, np.ndarray, int, int]:
@@ -832,10 +863,13 @@ class SparseWorker:
         self._l = lst
         self.keys = _u32(keys)
 
-    def run(self, d_res):
-        """ListUnion_sparse (test/adsfilter_q_all.cu:64-105): writes the full
-        result bitmap into device buffer d_res (stride_bytes long)."""
-        _check(_lib.ab_list_union_sparse(self._l.ctx._h, _ptr(self.keys), self.keys.size, _ptr(d_res)))
+    def run(self, d_res, res_len_bytes: Optional[int] = None):
+        """ListUnion_sparse (test/adsfilter_q_all.cu:64-105): writes the result
+        bitmap into device buffer d_res, min(its size, stride_bytes) bytes; a
+        torch tensor's size is its own, a raw address needs res_len_bytes
+        (at least ceil(nvalues / 8))."""
+        n = _buf_bytes(d_res, res_len_bytes)
+        _check(_lib.ab_list_union_sparse(self._l.ctx._h, _ptr(self.keys), self.keys.size, _ptr(d_res), n))
 
 
 class DenseWorker:
@@ -846,9 +880,11 @@ class DenseWorker:
         self.bitmap = np.ascontiguousarray(np.frombuffer(bytes(bitmap), dtype=np.uint8)[:nbytes]) \
             if not isinstance(bitmap, np.ndarray) else np.ascontiguousarray(bitmap.view(np.uint8)[:nbytes])
 
-    def run(self, d_res):
-        """ListUnion_dense (test/adsfilter_q_all.cu:26-61)."""
-        _check(_lib.ab_list_union_dense(self._l.ctx._h, self.bitmap.size, _ptr(self.bitmap), _ptr(d_res)))
+    def run(self, d_res, res_len_bytes: Optional[int] = None):
+        """ListUnion_dense (test/adsfilter_q_all.cu:26-61); d_res as in
+        SparseWorker.run."""
+        n = _buf_bytes(d_res, res_len_bytes)
+        _check(_lib.ab_list_union_dense(self._l.ctx._h, self.bitmap.size, _ptr(self.bitmap), _ptr(d_res), n))
 
 
 class BatchWorker:

@@ -374,16 +374,11 @@ extern "C" ab_status ab_index_load(const char *path, int host_only, ab_index **o
         set_error(std::string("ab_index_load: truncated or corrupt file: ") + path);
         return AB_IO_ERROR;
     }
-    // structural checks the kernels rely on
-    bool good = h.seg8.size() == ((size_t)h.nkeys + 1) * kSegStride && h.tile_blk.size() == (size_t)h.ntiles + 1 &&
-                h.vkeys.size() >= 64 && h.tile_shift >= 8 && h.tile_shift <= 16 &&
-                (uint64_t)h.ntiles == (((uint64_t)h.nvalues + (1ull << h.tile_shift) - 1) >> h.tile_shift);
-    for (int t = 0; t < kTypes && good; ++t)
-        good = h.pay[t].size() == h.hdr[t].size() * kPayloadBytes[t] && h.vhdr[t].size() == h.hdr[t].size() &&
-               h.vpay[t].size() == h.pay[t].size() && h.seg8[(size_t)h.nkeys * kSegStride + t] == h.hdr[t].size();
-    if (good && !h.tile_blk.empty()) good = h.tile_blk.back() == h.vblk.size();
-    if (!good) {
-        set_error(std::string("ab_index_load: inconsistent index file: ") + path);
+    // the checksum only detects accidents: every invariant the kernels rely
+    // on is checked as well, so a malformed file never reaches them
+    std::string why;
+    if (!validate_index(h, why)) {
+        set_error(std::string("ab_index_load: inconsistent index file (") + why + "): " + path);
         return AB_IO_ERROR;
     }
     AB_TRY(finish_index(ix, host_only != 0));
@@ -1066,21 +1061,54 @@ extern "C" ab_status ab_query(ab_ctx *c, const uint32_t *h_keys, uint32_t n, int
     return run_batch(c, a);
 }
 
-extern "C" ab_status ab_list_union_sparse(ab_ctx *c, const uint32_t *h_keys, uint32_t n, uint32_t *d_res)
+// A caller's result buffer of res_len bytes (at least ceil(nvalues / 8), as the
+// reference's callers size it, test/adsfilter_q_all.cu:361-368): the full
+// stride goes straight into it when it has room, else into the context's
+// bitmap scratch, from which exactly res_len bytes are copied.
+static ab_status result_target(ab_ctx *c, const char *fn, uint32_t *d_res, uint64_t res_len, uint32_t **target)
+{
+    const uint64_t need = ((uint64_t)c->ix->nvalues + 7) / 8;
+    if (res_len < need) {
+        set_error(std::string(fn) + ": res_len_bytes smaller than ceil(nvalues / 8)");
+        return AB_INVALID_ARG;
+    }
+    *target = d_res;
+    if (res_len < c->ix->stride) {
+        AB_TRY(c->bm.ensure(c->ix->stride));
+        *target = (uint32_t *)c->bm.p;
+    }
+    return AB_OK;
+}
+
+static ab_status result_finish(ab_ctx *c, uint32_t *d_res, uint64_t res_len, const uint32_t *target)
+{
+    if (target != d_res)
+        HIP_TRY(hipMemcpyAsync(d_res, target, res_len, hipMemcpyDeviceToDevice, c->stream));
+    return AB_OK;
+}
+
+extern "C" ab_status ab_list_union_sparse(ab_ctx *c, const uint32_t *h_keys, uint32_t n, uint32_t *d_res,
+                                          uint64_t res_len_bytes)
 {
     if (!c || (n && !h_keys) || !d_res) {
         set_error("ab_list_union_sparse: bad argument");
         return AB_INVALID_ARG;
     }
-    return ab_query(c, h_keys, n, AB_PATH_SPARSE, d_res, nullptr, nullptr, nullptr);
+    uint32_t *t = nullptr;
+    AB_TRY(result_target(c, "ab_list_union_sparse", d_res, res_len_bytes, &t));
+    AB_TRY(ab_query(c, h_keys, n, AB_PATH_SPARSE, t, nullptr, nullptr, nullptr));
+    return result_finish(c, d_res, res_len_bytes, t);
 }
 
-extern "C" ab_status ab_list_union_dense(ab_ctx *c, uint64_t len_bytes, const uint8_t *h_bitmap, uint32_t *d_res)
+extern "C" ab_status ab_list_union_dense(ab_ctx *c, uint64_t len_bytes, const uint8_t *h_bitmap, uint32_t *d_res,
+                                         uint64_t res_len_bytes)
 {
     if (!c || (len_bytes && !h_bitmap) || !d_res) {
         set_error("ab_list_union_dense: bad argument");
         return AB_INVALID_ARG;
     }
+    uint32_t *t = nullptr;
+    AB_TRY(result_target(c, "ab_list_union_dense", d_res, res_len_bytes, &t));
     ab_index *ix = c->ix;
     const uint64_t qbytes = ((uint64_t)ix->nkeys + 31) / 32 * 4;
     const uint64_t nb = std::min<uint64_t>(len_bytes, qbytes);
@@ -1096,10 +1124,10 @@ extern "C" ab_status ab_list_union_dense(ab_ctx *c, uint64_t len_bytes, const ui
     g.B = 1;
     g.q[0] = 0;
     const uint64_t w = ix->stride / 4;
-    HIP_TRY(launch_dense_tiles(kModeBitmap, c->dense_threads, ix->dev, c->qbm.p, (const uint32_t *)c->qbm.p, g, d_res, w,
+    HIP_TRY(launch_dense_tiles(kModeBitmap, c->dense_threads, ix->dev, c->qbm.p, (const uint32_t *)c->qbm.p, g, t, w,
                                w, false, c->stream));
     HIP_TRY(hipMemsetAsync(c->qbm.p, 0, qbytes, c->stream));
-    return AB_OK;
+    return result_finish(c, d_res, res_len_bytes, t);
 }
 
 extern "C" ab_status ab_list_union_dense_batch(ab_ctx *c, uint32_t B, uint64_t len_bytes, const uint8_t *d_bitmaps,

@@ -12,6 +12,9 @@
 #include <memory>
 #include <unordered_set>
 
+#include <dirent.h>
+#include <sys/stat.h>
+
 #include "internal.h"
 
 namespace ab {
@@ -342,6 +345,55 @@ extern "C" int64_t ab_read_query(const char *path, uint32_t *keys, uint64_t cap)
         ++n;
     });
     return n;
+}
+
+// FindQueryFiles (test/adsfilter_q_all.cu:211-231): regular files (symlinks
+// followed) of dir whose whole name matches query_vec_[0-9]+\.coo, as dir
+// joined with the name, sorted as byte strings.
+static bool is_query_name(const char *n)
+{
+    static const char kPre[] = "query_vec_", kSuf[] = ".coo";
+    const size_t len = strlen(n), pre = sizeof(kPre) - 1, suf = sizeof(kSuf) - 1;
+    if (len < pre + 1 + suf || strncmp(n, kPre, pre) != 0 || strcmp(n + len - suf, kSuf) != 0) return false;
+    for (size_t i = pre; i < len - suf; ++i)
+        if (n[i] < '0' || n[i] > '9') return false;
+    return true;
+}
+
+extern "C" int64_t ab_find_query_files(const char *dir, char *buf, uint64_t cap, uint64_t *need)
+{
+    if (need) *need = 0;
+    if (!dir) {
+        set_error("ab_find_query_files: NULL directory");
+        return -1;
+    }
+    DIR *d = opendir(dir);
+    if (!d) {
+        set_error(std::string("ab_find_query_files: cannot read directory ") + dir);
+        return -1;
+    }
+    std::string base(dir);
+    if (!base.empty() && base.back() != '/') base += '/';
+    std::vector<std::string> files;
+    while (struct dirent *e = readdir(d)) {
+        if (!is_query_name(e->d_name)) continue;
+        std::string p = base + e->d_name;
+        struct stat st;
+        if (stat(p.c_str(), &st) == 0 && S_ISREG(st.st_mode)) files.push_back(std::move(p));
+    }
+    closedir(d);
+    std::sort(files.begin(), files.end());
+    uint64_t total = 0;
+    for (const auto &f : files) total += f.size() + 1;
+    if (need) *need = total;
+    if (buf && cap >= total) {
+        char *o = buf;
+        for (const auto &f : files) {
+            memcpy(o, f.c_str(), f.size() + 1);
+            o += f.size() + 1;
+        }
+    }
+    return (int64_t)files.size();
 }
 
 extern "C" int64_t ab_read_coo(const char *path, uint32_t *keys, uint32_t *values, uint64_t cap,

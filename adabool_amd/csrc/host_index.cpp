@@ -347,6 +347,111 @@ ab_status build_host_index(const uint32_t *keys, const uint32_t *vals, uint64_t 
     return AB_OK;
 }
 
+// Values of one record (header h, payload p of type t) in emit order: the
+// header value, then payload bytes [0, kMinNum) always and later bytes up to
+// the first zero; a bitset's set bits.  f(value) per value.
+template <class F>
+static void for_record_values(int t, uint32_t h, const uint8_t *p, F &&f)
+{
+    if (t == AB_BITSET) {
+        for (int x = 0; x < 256; ++x)
+            if (p[x >> 3] & (1u << (x & 7))) f((uint64_t)(h & 0xffffff00u) | (uint32_t)x);
+        return;
+    }
+    f((uint64_t)h);
+    for (int i = 0; i < kPayloadBytes[t]; ++i) {
+        if (i >= kMinNum[t] && p[i] == 0) break;
+        f((uint64_t)(h & 0xffffff00u) | p[i]);
+    }
+}
+
+bool validate_index(const HostIndex &h, std::string &why)
+{
+    auto fail = [&](const std::string &m) {
+        why = m;
+        return false;
+    };
+    const uint64_t nk = h.nkeys;
+    if (h.seg8.size() != (nk + 1) * kSegStride) return fail("seg8 has the wrong size");
+    if (h.tile_shift < 8 || h.tile_shift > 16) return fail("value tile out of range");
+    if ((uint64_t)h.ntiles != (((uint64_t)h.nvalues + (1ull << h.tile_shift) - 1) >> h.tile_shift))
+        return fail("tile count does not match the value space");
+    if (h.tile_blk.size() != (size_t)h.ntiles + 1) return fail("tile block list has the wrong size");
+    if (h.vkeys.size() < 64) return fail("key stream too short");
+    for (int t = 0; t < kTypes; ++t)
+        if (h.pay[t].size() != h.hdr[t].size() * kPayloadBytes[t] || h.vhdr[t].size() != h.hdr[t].size() ||
+            h.vpay[t].size() != h.pay[t].size())
+            return fail("record arrays of a type disagree in size");
+    // per-key prefix counts: row 0 zero, every column non-decreasing, the
+    // all-type column the sum of the six, totals equal to the arrays
+    for (int c = 0; c < kSegStride; ++c)
+        if (h.seg8[c] != 0) return fail("seg8 row 0 is not zero");
+    for (uint64_t k = 0; k < nk; ++k) {
+        const uint32_t *a = &h.seg8[k * kSegStride], *b = a + kSegStride;
+        uint64_t sum = 0;
+        for (int c = 0; c < kSegStride; ++c)
+            if (b[c] < a[c]) return fail("seg8 is not non-decreasing");
+        for (int t = 0; t < kTypes; ++t) sum += b[t];
+        if (sum != b[6]) return fail("seg8 all-type prefix is not the sum of the types");
+    }
+    const uint32_t *last = &h.seg8[nk * kSegStride];
+    for (int t = 0; t < kTypes; ++t)
+        if (last[t] != h.hdr[t].size()) return fail("seg8 totals do not match the record arrays");
+    if (last[7] != (uint32_t)h.nnz) return fail("seg8 value total does not match the pair count");
+    // every value a record emits lies in the value space; a bitset header is
+    // window-aligned; a key's value count is what its records hold (the id
+    // bounds of admission rely on it)
+    for (int t = 0; t < kTypes; ++t) {
+        const uint64_t n = h.hdr[t].size();
+        for (uint64_t i = 0; i < n; ++i) {
+            bool bad = false;
+            auto chk = [&](uint64_t v) { bad = bad || v >= h.nvalues; };
+            if (t == AB_BITSET && ((h.hdr[t][i] & 0xffu) || (h.vhdr[t][i] & 0xffu)))
+                return fail("bitset header is not window-aligned");
+            for_record_values(t, h.hdr[t][i], &h.pay[t][i * kPayloadBytes[t]], chk);
+            for_record_values(t, h.vhdr[t][i], &h.vpay[t][i * kPayloadBytes[t]], chk);
+            if (bad) return fail("a record holds a value outside the value space");
+        }
+    }
+    {
+        uint64_t idx[kTypes] = {0};
+        for (uint64_t k = 0; k < nk; ++k) {
+            const uint32_t *a = &h.seg8[k * kSegStride], *b = a + kSegStride;
+            uint64_t nvals = 0;
+            for (int t = 0; t < kTypes; ++t)
+                for (; idx[t] < b[t]; ++idx[t])
+                    for_record_values(t, h.hdr[t][idx[t]], &h.pay[t][idx[t] * kPayloadBytes[t]],
+                                      [&](uint64_t) { ++nvals; });
+            if (nvals != (uint64_t)(b[7] - a[7])) return fail("a key's value count disagrees with its records");
+        }
+    }
+    // value-tiled blocks: fields in range, keys and records inside their
+    // arrays, every record's window inside the block's tile, every key < nkeys
+    if (h.tile_blk[0] != 0 || h.tile_blk.back() != h.vblk.size()) return fail("tile block list ends are wrong");
+    for (uint32_t tl = 0; tl < h.ntiles; ++tl)
+        if (h.tile_blk[tl + 1] < h.tile_blk[tl]) return fail("tile block list is not non-decreasing");
+    for (uint32_t tl = 0; tl < h.ntiles; ++tl) {
+        for (uint32_t bi = h.tile_blk[tl]; bi < h.tile_blk[tl + 1]; ++bi) {
+            const BlockMeta &m = h.vblk[bi];
+            const uint32_t t = m.tw & 15, wl = (m.tw >> 4) & 3, nv = m.tw >> 8;
+            if (t >= (uint32_t)kTypes || wl > 2 || (m.tw & 0xc0u) || nv == 0 || nv > kBlockRecs)
+                return fail("key-stream block descriptor out of range");
+            if ((uint64_t)m.rec_base + nv > h.vhdr[t].size()) return fail("key-stream block records out of range");
+            const uint64_t off = (uint64_t)m.off_kb << 10, w = 1ull << wl;
+            if (off + (uint64_t)kBlockRecs * w > h.vkeys.size()) return fail("key-stream block bytes out of range");
+            for (uint32_t r = 0; r < nv; ++r) {
+                const uint8_t *p = &h.vkeys[off + ((uint64_t)(r & 63) * kSlots + (r >> 6)) * w];
+                uint32_t d = 0;
+                for (uint32_t q = 0; q < w; ++q) d |= (uint32_t)p[q] << (8 * q);
+                if ((uint64_t)m.start_key + d >= nk) return fail("key-stream key outside the key space");
+                if ((h.vhdr[t][(uint64_t)m.rec_base + r] >> h.tile_shift) != tl)
+                    return fail("a block's record lies outside its value tile");
+            }
+        }
+    }
+    return true;
+}
+
 void host_features(const uint32_t *seg8, uint32_t nkeys, uint32_t nvalues, uint64_t R,
                    const uint32_t *keys, uint32_t n, float *f, uint64_t *m_out)
 {

@@ -75,6 +75,11 @@ uint32_t auto_tile_shift(uint32_t nvalues);
 void host_features(const uint32_t *seg8, uint32_t nkeys, uint32_t nvalues, uint64_t R,
                    const uint32_t *keys, uint32_t n, float *f, uint64_t *m);
 
+// Every invariant the kernels rely on, checked over a loaded index in one
+// pass (index files from outside the process); false with a reason if any
+// fails.
+bool validate_index(const HostIndex &h, std::string &why);
+
 uint64_t stride_bytes(uint32_t nvalues);
 int default_threads();
 // COO text file -> pairs in file order (ReadCOO rules), parsed in parallel.

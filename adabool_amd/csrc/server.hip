@@ -207,17 +207,21 @@ void serve(ab_server *s)
                     s->dense_queries += pth[i] == AB_PATH_DENSE;
                 }
                 if (s->cfg.deliver == 1) {
-                    // ids -> pinned host mirror: one copy per run of adjacent reservations
+                    // ids -> pinned host mirror: one copy per run of members
+                    // whose reservations follow each other with no gap, so a
+                    // copy never spans arena space that another request (on
+                    // another slot's stream) may still be writing
                     lk.unlock();
                     hipStream_t st = sl.ctx->stream;
                     hipError_t e = hipSuccess;
                     size_t i = 0;
                     while (i < sl.members.size() && e == hipSuccess) {
                         uint64_t a = sl.members[i]->off, b = a + sl.members[i]->count;
+                        uint64_t res_end = a + Arena::round(sl.members[i]->ub);  // end of the run's reservations
                         size_t j = i + 1;
-                        while (j < sl.members.size() && sl.members[j]->off >= a &&
-                               sl.members[j]->off <= ((b + 3) & ~3ull) + 64) {
-                            b = std::max(b, sl.members[j]->off + sl.members[j]->count);
+                        while (j < sl.members.size() && sl.members[j]->off == res_end) {
+                            b = sl.members[j]->off + sl.members[j]->count;
+                            res_end = sl.members[j]->off + Arena::round(sl.members[j]->ub);
                             ++j;
                         }
                         if (b > a)

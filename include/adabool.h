@@ -227,7 +227,8 @@ ab_status ab_ctx_timing(ab_ctx *ctx, double *ms, uint64_t *launches, int reset);
 
 /* ---- single query --------------------------------------------------------- */
 /* One query of n host keys (duplicates and keys >= nkeys allowed).  Writes
- * the bitmap (d_bitmap, may be NULL for ids only), the ascending ids (d_ids,
+ * the bitmap (d_bitmap, ab_result_stride_bytes() bytes, may be NULL for ids
+ * only), the ascending ids (d_ids,
  * capacity nvalues, may be NULL) and the id count (d_count, device u32, may be
  * NULL).  h_chosen (may be NULL) receives the path that ran.
  * Replaces Prepare(keys) / Prepare(len, bitmap) + ListUnion_sparse /
@@ -236,17 +237,21 @@ ab_status ab_ctx_timing(ab_ctx *ctx, double *ms, uint64_t *launches, int reset);
 ab_status ab_query(ab_ctx *ctx, const uint32_t *h_keys, uint32_t n, int path,
                    uint32_t *d_bitmap, uint32_t *d_ids, uint32_t *d_count, int *h_chosen);
 
-/* Sparse method on host keys into d_res (full stride written):
+/* Sparse method on host keys into d_res, a device buffer of res_len_bytes
+ * bytes, at least ceil(nvalues / 8) (the size the reference's callers
+ * allocate, test/adsfilter_q_all.cu:361-368; smaller is AB_INVALID_ARG).
+ * min(res_len_bytes, ab_result_stride_bytes()) bytes are written, never more:
  * Prepare(keys) + ListUnion_sparse (src/AdaBool/inverted_list.h:481-527,
  * test/adsfilter_q_all.cu:64-105). */
 ab_status ab_list_union_sparse(ab_ctx *ctx, const uint32_t *h_keys, uint32_t n,
-                               uint32_t *d_res);
+                               uint32_t *d_res, uint64_t res_len_bytes);
 
 /* Dense method on a host key bitmap of len_bytes bytes (bit k set = key k in
- * the query): Prepare(len, bitmap) + ListUnion_dense
+ * the query) into d_res, sized and written as for ab_list_union_sparse:
+ * Prepare(len, bitmap) + ListUnion_dense
  * (src/AdaBool/inverted_list.h:530-543, test/adsfilter_q_all.cu:26-61). */
 ab_status ab_list_union_dense(ab_ctx *ctx, uint64_t len_bytes, const uint8_t *h_bitmap,
-                              uint32_t *d_res);
+                              uint32_t *d_res, uint64_t res_len_bytes);
 
 /* Batched dense method on B device bitmaps laid out query-major, each
  * len_bytes bytes (a multiple of 4); query i's result goes to
@@ -456,6 +461,14 @@ void ab_dataset_free(ab_dataset *d);
 /* ReadQuery (test/adsfilter_q_all.cu:164-193): returns the key count (or -1)
  * and fills keys when non-NULL. */
 int64_t ab_read_query(const char *path, uint32_t *keys, uint64_t cap);
+/* FindQueryFiles (test/adsfilter_q_all.cu:211-231): the regular files of dir
+ * (symlinks followed) whose whole name matches query_vec_[0-9]+\.coo, as
+ * dir + "/" + name, sorted lexicographically as byte strings (so
+ * query_vec_10.coo comes before query_vec_2.coo).  Returns the file count, or
+ * -1 when dir cannot be read (the reference printed the error and returned no
+ * files).  *need (may be NULL) gets the bytes of all paths, NUL-terminated one
+ * after another; they are written to buf only when cap holds them all. */
+int64_t ab_find_query_files(const char *dir, char *buf, uint64_t cap, uint64_t *need);
 /* ReadCOO into caller buffers (two-call: keys == NULL counts). */
 int64_t ab_read_coo(const char *path, uint32_t *keys, uint32_t *values, uint64_t cap,
                     uint32_t *max_key, uint32_t *max_value);

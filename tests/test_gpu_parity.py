@@ -236,6 +236,35 @@ def test_reference_api(c1):
         assert np.array_equal(out[b, :sw], csr.union(qs[b])[0]), f"batch sparse {b}"
 
 
+@pytest.mark.parametrize("nvalues", [100_000, 100_003, 99_985])
+def test_reference_sized_results_stay_in_bounds(nvalues):
+    """The reference's callers allocate ceil(nvalues / 8) result bytes
+    (test/adsfilter_q_all.cu:361-368); the single-query entry points write
+    exactly that much into such a buffer and nothing past it."""
+    import torch
+
+    k, v = ab.gen_pairs(0xADAB0150, nvalues, 5_000, lo=5, hi=15, zipf_s=1.0)
+    lst = ab.IvfList().BuildFrom((k, v), nkeys=5_000, nvalues=nvalues)
+    csr = oracle.CSR(k, v, 5_000, nvalues)
+    need = (nvalues + 7) // 8
+    qk, qo = ab.gen_queries(0xADAB0151, 5_000, 2, lo=64, hi=64)
+    qs = [qk[qo[i]:qo[i + 1]] for i in range(2)]
+    qb = np.zeros((5_000 + 7) // 8, np.uint8)
+    for x in qs[1]:
+        qb[x // 8] |= np.uint8(1 << (x % 8))
+    for which, run, q in (("sparse", lambda d: lst.Prepare(qs[0]).run(d[:need]), qs[0]),
+                          ("dense", lambda d: lst.PrepareBitmap(qb.size, qb).run(d[:need]), qs[1])):
+        big = torch.full((need + 64,), 0xA5, dtype=torch.uint8, device="cuda")
+        run(big)
+        lst.sync()
+        got = big.cpu().numpy()
+        want = csr.union(q)[0].view(np.uint8)[:need]
+        assert np.array_equal(got[:need], want), f"{which}: bitmap differs (nvalues {nvalues})"
+        assert (got[need:] == 0xA5).all(), f"{which}: wrote past ceil(nvalues / 8) bytes (nvalues {nvalues})"
+    with pytest.raises(ab.AdaBoolError):
+        lst.Prepare(qs[0]).run(torch.zeros(need - 1, dtype=torch.uint8, device="cuda"))
+
+
 def test_features_device_matches_host(c1):
     import torch
 

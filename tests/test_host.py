@@ -179,6 +179,37 @@ def test_text_loaders_match_oracle():
             ab.read_query(os.path.join(d, "missing.coo"))
 
 
+def test_find_query_files_order_and_filter(capsys):
+    """FindQueryFiles (test/adsfilter_q_all.cu:211-231): whole-name match of
+    query_vec_<digits>.coo over regular files (symlinks followed), sorted as
+    strings, so _10 runs before _2.  Known answer written out by hand."""
+    with tempfile.TemporaryDirectory() as d:
+        for n in ["query_vec_2.coo", "query_vec_10.coo", "query_vec_1.coo", "query_vec_007.coo",
+                  "query_vec_.coo", "query_vec_3.coo.bak", "xquery_vec_4.coo", "query_vec_5.COO",
+                  "query_vec_1a.coo", "query_vec_-1.coo", "filter.coo"]:
+            open(os.path.join(d, n), "w").write("1\n")
+        os.mkdir(os.path.join(d, "query_vec_6.coo"))                       # a directory: skipped
+        os.symlink(os.path.join(d, "filter.coo"), os.path.join(d, "query_vec_7.coo"))  # to a file: kept
+        os.symlink(os.path.join(d, "nowhere"), os.path.join(d, "query_vec_8.coo"))     # dangling: skipped
+        want = [os.path.join(d, n) for n in
+                ["query_vec_007.coo", "query_vec_1.coo", "query_vec_10.coo", "query_vec_2.coo", "query_vec_7.coo"]]
+        assert ab.find_query_files(d) == want
+        assert ab.find_query_files(d + "/") == want  # no doubled separator
+        assert ab.find_query_files(os.path.join(d, "missing")) == []
+        assert "Error reading directory" in capsys.readouterr().err
+        empty = os.path.join(d, "empty")
+        os.mkdir(empty)
+        assert ab.find_query_files(empty) == []
+        # the C ABI's sizing contract: nothing written into a short buffer
+        lib = C.CDLL(ab.lib_path())
+        lib.ab_find_query_files.restype = C.c_int64
+        lib.ab_find_query_files.argtypes = [C.c_char_p, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
+        need = C.c_uint64()
+        buf = C.create_string_buffer(b"\x7f" * 16, 16)
+        assert lib.ab_find_query_files(d.encode(), buf, 16, C.byref(need)) == 5
+        assert need.value == sum(len(p) + 1 for p in want) and buf.raw == b"\x7f" * 16
+
+
 def test_large_coo_parallel_parse_matches_oracle():
     """A text file above the parallel-parse size (chunks cut at line starts)
     with comments, blank lines, CRLF, tabs and bad lines spread through it."""

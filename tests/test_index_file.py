@@ -63,6 +63,107 @@ def test_damaged_files_are_refused(tmp_path, damage):
     assert e.value.status == 5  # AB_IO_ERROR
 
 
+# ---- malformed files with a valid checksum --------------------------------
+# File layout (ab_index_save): a 40-byte header, then per section a u64 byte
+# count, the bytes and zero padding to 8, then the 64-bit FNV-1a checksum of
+# everything before it.  Sections: seg8, then per type (hdr, pay, vhdr, vpay),
+# then vblk, vkeys, tile_blk.
+_NSEC = 1 + 4 * ab.NTYPES + 3
+
+
+def _fnv1a(data: bytes) -> int:
+    h = 1469598103934665603
+    for b in data:
+        h = ((h ^ b) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _sections(data: bytes):
+    out, pos = [], 40
+    for _ in range(_NSEC):
+        n = int(np.frombuffer(data, np.uint64, 1, pos)[0])
+        out.append((pos + 8, n))
+        pos += 8 + n + (8 - n % 8) % 8
+    return out
+
+
+def _resave(path, data: bytearray):
+    body = bytes(data[:-8])
+    open(path, "wb").write(body + np.uint64(_fnv1a(body)).tobytes())
+
+
+def _sec_u32(data, secs, i):
+    off, n = secs[i]
+    return np.frombuffer(data, np.uint32, n // 4, off), off
+
+
+@pytest.mark.parametrize("damage", ["km_header_value", "km_payload_value", "seg8_order", "vblk_records",
+                                    "vblk_key_bytes", "vt_header_tile", "vt_key_range", "tile_blk_order",
+                                    "bitset_align"])
+def test_malformed_file_with_valid_checksum_is_refused(tmp_path, damage):
+    """Checksums detect accidents only: every invariant the kernels rely on is
+    checked at load, so a file that breaks one is refused even when its
+    checksum was recomputed (values, keys, record and key-stream offsets, tile
+    membership, prefix order)."""
+    nvalues, nkeys = 20000, 3000  # 20000 = 78 * 256 + 32: the last window is partial
+    k, v = _filter()
+    ix = ab.Index(k, v, nkeys, nvalues, host_only=True)
+    p = str(tmp_path / "m.abidx")
+    ix.save(p)
+    data = bytearray(open(p, "rb").read())
+    secs = _sections(bytes(data))
+    # pristine re-save loads (the checksum writer here matches the library's)
+    _resave(p, data)
+    ab.Index.load(p, host_only=True)
+
+    def put_u32(off, val):
+        data[off:off + 4] = np.uint32(val).tobytes()
+
+    hdr = lambda t: 1 + 4 * t      # noqa: E731
+    pay = lambda t: 2 + 4 * t      # noqa: E731
+    vhdr = lambda t: 3 + 4 * t     # noqa: E731
+    vblk, vkeys, tblk = 1 + 4 * ab.NTYPES, 2 + 4 * ab.NTYPES, 3 + 4 * ab.NTYPES
+    if damage == "km_header_value":
+        _, off = _sec_u32(bytes(data), secs, hdr(0))
+        put_u32(off, nvalues + 5)
+    elif damage == "km_payload_value":
+        # an ARRAY4 record moved into the partial last window, its first
+        # payload byte past the end of the value space
+        _, off = _sec_u32(bytes(data), secs, hdr(1))
+        put_u32(off, 78 * 256 + 1)
+        poff, _ = secs[pay(1)]
+        data[poff] = 250
+    elif damage == "seg8_order":
+        a, off = _sec_u32(bytes(data), secs, 0)
+        row = int(np.argmax(a.reshape(-1, 8)[:, 7] > 0))
+        put_u32(off + (row * 8 + 7) * 4, 0xFFFFFFF0)  # the next row's value prefix is then smaller
+    elif damage == "vblk_records":
+        off, _ = secs[vblk]
+        put_u32(off + 12, 0x7FFFFFFF)          # rec_base of block 0
+    elif damage == "vblk_key_bytes":
+        off, _ = secs[vblk]
+        put_u32(off + 4, 0x00FFFFFF)           # off_kb of block 0
+    elif damage == "vt_header_tile":
+        t = int(np.argmax([len(_sec_u32(bytes(data), secs, vhdr(t))[0]) > 0 for t in range(ab.NTYPES)]))
+        a, off = _sec_u32(bytes(data), secs, vhdr(t))
+        # a record of tile 0 given a header in the last tile (still < nvalues)
+        put_u32(off, (int(a[0]) & 0xFF) | (nvalues - 256) & 0xFFFFFF00 if t != 5 else (nvalues - 256) & 0xFFFFFF00)
+    elif damage == "vt_key_range":
+        off, _ = secs[vblk]
+        put_u32(off, nkeys)                    # start_key of block 0: every key >= nkeys
+    elif damage == "tile_blk_order":
+        a, off = _sec_u32(bytes(data), secs, tblk)
+        put_u32(off + 4, int(a[-1]) + 1)
+    else:
+        a, off = _sec_u32(bytes(data), secs, hdr(5))
+        assert a.size > 0
+        put_u32(off, int(a[0]) | 3)
+    _resave(p, data)
+    with pytest.raises(ab.AdaBoolError) as e:
+        ab.Index.load(p, host_only=True)
+    assert e.value.status == 5, str(e.value)  # AB_IO_ERROR
+
+
 def test_missing_file():
     with pytest.raises(ab.AdaBoolError):
         ab.Index.load("/nonexistent/index.abidx", host_only=True)
