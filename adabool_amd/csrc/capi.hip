@@ -1258,9 +1258,8 @@ static ab_status query_batch(const char *fn, ab_ctx *c, const uint32_t *d_keys, 
         a.h_M = M;
         return run_batch(c, a);
     }
-    // the fused path's grids put queries on the y dimension (at most 65535)
-    // and its persistent kernel is built for 256 and 512 threads
-    if (c->batch_mode == 0 || nq == 0 || nq > 65535 || c->dense_threads < 256) return run_batch(c, a);
+    // the persistent kernel is built for 256 and 512 threads
+    if (c->batch_mode == 0 || nq == 0 || c->dense_threads < 256) return run_batch(c, a);
     return run_batch_fused(c, a);
 }
 

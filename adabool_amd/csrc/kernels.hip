@@ -895,15 +895,22 @@ __global__ __launch_bounds__(256) void k_zero_sparse(uint32_t *bm, uint64_t stri
     if (sel[q] != AB_PATH_SPARSE) return;
     const uint64_t w0 = (uint64_t)blockIdx.x * kZeroChunk;
     const uint64_t w1 = w0 + kZeroChunk < words ? w0 + kZeroChunk : words;
-    uint32_t *dst = bm + q * stride;
+    uint32_t *dst = bm + (uint64_t)q * stride;
     for (uint64_t i = w0 + threadIdx.x; i < w1; i += 256) dst[i] = 0;
 }
+
+// queries go on the grid's y dimension, in slices of at most 65535
+constexpr uint32_t kMaxGridY = 65535;
 
 void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq, hipStream_t s)
 {
     if (!nq || !words) return;
     const uint32_t gx = (uint32_t)((words + kZeroChunk - 1) / kZeroChunk);
-    hipLaunchKernelGGL(k_zero_sparse, dim3(gx, nq), dim3(256), 0, s, bm, stride, words, sel);
+    for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
+        const uint32_t m = nq - q0 < kMaxGridY ? nq - q0 : kMaxGridY;
+        hipLaunchKernelGGL(k_zero_sparse, dim3(gx, m), dim3(256), 0, s, bm + (uint64_t)q0 * stride, stride, words,
+                           sel + q0);
+    }
 }
 
 template <int MODE>
@@ -938,20 +945,25 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
                         void *masks, uint64_t mask_stride, uint32_t *anybms, uint64_t any_stride, uint32_t nkeys,
                         int clear, hipStream_t s)
 {
-    if (!nq) return;
-    dim3 grid(8, nq);
-    if (mode == kModeMask32)
-        hipLaunchKernelGGL(k_mask_groups<kModeMask32>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks, mask_stride,
-                           anybms, any_stride, nkeys, clear);
-    else if (mode == kModeMask64)
-        hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks, mask_stride,
-                           anybms, any_stride, nkeys, clear);
-    else if (mode == kModeMask128)
-        hipLaunchKernelGGL(k_mask_groups<kModeMask128>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks,
-                           mask_stride, anybms, any_stride, nkeys, clear);
-    else
-        hipLaunchKernelGGL(k_mask_groups<kModeMask256>, grid, dim3(256), 0, s, keys, qoff, qgroup, masks,
-                           mask_stride, anybms, any_stride, nkeys, clear);
+    for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
+        // a slice's kernels read qoff[q], qoff[q + 1] and qgroup[q] relative to its first query
+        const uint32_t m = nq - q0 < kMaxGridY ? nq - q0 : kMaxGridY;
+        const dim3 grid(8, m);
+        const uint64_t *qo = qoff + q0;
+        const uint32_t *qg = qgroup + q0;
+        if (mode == kModeMask32)
+            hipLaunchKernelGGL(k_mask_groups<kModeMask32>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
+                               anybms, any_stride, nkeys, clear);
+        else if (mode == kModeMask64)
+            hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
+                               anybms, any_stride, nkeys, clear);
+        else if (mode == kModeMask128)
+            hipLaunchKernelGGL(k_mask_groups<kModeMask128>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
+                               anybms, any_stride, nkeys, clear);
+        else
+            hipLaunchKernelGGL(k_mask_groups<kModeMask256>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
+                               anybms, any_stride, nkeys, clear);
+    }
 }
 
 __global__ void k_set_u32(uint32_t *p, uint32_t v)
@@ -1077,9 +1089,6 @@ __global__ __launch_bounds__(256) void k_popcount(const uint32_t *__restrict__ b
     __syncthreads();
     if (threadIdx.x == 0) ccnt[(uint64_t)q * nchunk + c] = sh[0] + sh[1] + sh[2] + sh[3];
 }
-
-// queries go on the grid's y dimension, in slices of at most 65535
-constexpr uint32_t kMaxGridY = 65535;
 
 void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, uint32_t *ccnt, hipStream_t s)
 {

@@ -59,6 +59,10 @@ def lib():
         "or_sparse_union": (C.c_uint64, [_VP, _VP, C.c_uint32, C.c_uint32, _VP]),
         "or_union_batch": (C.c_double, [_VP, _VP, C.c_uint32, _VP, _VP, C.c_uint32, C.c_uint32, C.c_int, _VP,
                                         _VP, C.POINTER(C.c_uint64)]),
+        "or_union_batch_digest": (None, [_VP, _VP, C.c_uint32, _VP, _VP, C.c_uint32, C.c_uint32, C.c_int, _VP,
+                                         _VP, _VP]),
+        "or_bitmap_digest": (None, [_VP, C.c_uint64, C.c_uint64, C.c_uint64, C.c_int, _VP]),
+        "or_ids_digest": (None, [_VP, _VP, _VP, C.c_uint64, C.c_int, _VP, _VP]),
         "or_stats": (None, [_VP, C.c_uint64, _VP]),
         "or_read_coo": (C.c_int64, [C.c_char_p, _VP, _VP, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "or_read_query": (C.c_int64, [C.c_char_p, _VP]),
@@ -127,6 +131,44 @@ class CSR:
         t = lib().or_union_batch(_p(self.off), _p(self.vals), self.nkeys, _p(k), _p(off), nq, self.nvalues,
                                  nthreads, _p(counts), _p(lat), C.byref(cs))
         return t, counts[:nq], lat[:nq], cs.value
+
+
+    def union_batch_digest(self, qkeys, qoff, nthreads: int = 1):
+        """Per query: (count, bitmap digest, id digest) of the oracle's union
+        (definitions in oracle.h)."""
+        k = _u32(qkeys)
+        off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
+        nq = off.size - 1
+        c = np.zeros(max(nq, 1), np.uint64)
+        db = np.zeros(max(nq, 1), np.uint64)
+        di = np.zeros(max(nq, 1), np.uint64)
+        lib().or_union_batch_digest(_p(self.off), _p(self.vals), self.nkeys, _p(k), _p(off), nq, self.nvalues,
+                                    nthreads, _p(c), _p(db), _p(di))
+        return c[:nq], db[:nq], di[:nq]
+
+
+def bitmap_digest(bitmaps, nwords: int = 0, nthreads: int = 1) -> np.ndarray:
+    """Digest of each row of a [nq, stride] u32 array (its first nwords words,
+    default all)."""
+    b = np.ascontiguousarray(np.asarray(bitmaps).view(np.uint32))
+    if b.ndim == 1:
+        b = b.reshape(1, -1)
+    out = np.zeros(max(b.shape[0], 1), np.uint64)
+    lib().or_bitmap_digest(_p(b), b.shape[1], nwords or b.shape[1], b.shape[0], nthreads, _p(out))
+    return out[:b.shape[0]]
+
+
+def ids_digest(ids, off, counts, nthreads: int = 1) -> Tuple[np.ndarray, np.ndarray]:
+    """(digest, strictly ascending) of each id list ids[off[q] : off[q] + counts[q]]."""
+    a = np.ascontiguousarray(np.asarray(ids).view(np.uint32))
+    o = np.ascontiguousarray(np.asarray(off, dtype=np.uint64))
+    c = np.ascontiguousarray(np.asarray(counts, dtype=np.uint32))
+    if o.size and int((o + c).max()) > a.size:
+        raise ValueError("an id range lies outside the array")
+    out = np.zeros(max(o.size, 1), np.uint64)
+    asc = np.zeros(max(o.size, 1), np.uint8)
+    lib().or_ids_digest(_p(a), _p(o), _p(c), o.size, nthreads, _p(out), _p(asc))
+    return out[:o.size], asc[:o.size].astype(bool)
 
 
 def scan_coo(keys, vals, q, nvalues: int) -> np.ndarray:

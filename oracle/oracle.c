@@ -525,6 +525,121 @@ double or_union_batch(const uint64_t *off, const uint32_t *vals, uint32_t nkeys,
 }
 
 /* ------------------------------------------------------------------------ */
+/* per-query digests (checkers for large batches)                            */
+/* ------------------------------------------------------------------------ */
+
+static inline uint64_t dg_wt(uint64_t w) { return ((w + 1) * 0x9E3779B97F4A7C15ull) | 1ull; }
+static inline uint64_t dg_mix(uint32_t x)
+{
+    return ((uint64_t)x * 0x9E3779B1ull) ^ ((uint64_t)(x ^ 0x5BD1E995u) * 0x27D4EB2Full);
+}
+
+static uint64_t dg_bitmap(const uint32_t *bm, uint64_t nwords)
+{
+    uint64_t h = 0;
+    for (uint64_t w = 0; w < nwords; ++w) h += (uint64_t)bm[w] * dg_wt(w);
+    return h;
+}
+
+static uint64_t dg_ids(const uint32_t *ids, uint64_t n, uint8_t *asc)
+{
+    uint64_t h = 0;
+    uint8_t ok = 1;
+    for (uint64_t i = 0; i < n; ++i) {
+        h += dg_mix(ids[i]);
+        if (i && ids[i] <= ids[i - 1]) ok = 0;
+    }
+    if (asc) *asc = ok;
+    return h;
+}
+
+typedef struct {
+    int kind; /* 0 union, 1 bitmaps, 2 ids */
+    const uint64_t *off;
+    const uint32_t *vals;
+    uint32_t nkeys, nvalues;
+    const uint32_t *qkeys;
+    const uint64_t *qoff;
+    const uint32_t *data;
+    uint64_t stride, nwords;
+    const uint64_t *doff;
+    const uint32_t *dcnt;
+    uint64_t nq;
+    uint64_t *counts, *d0, *d1;
+    uint8_t *asc;
+    volatile uint64_t next;
+} digest_job;
+
+static void *digest_worker(void *arg)
+{
+    digest_job *J = (digest_job *)arg;
+    uint32_t *bm = NULL, *ids = NULL;
+    uint64_t nw = 0;
+    if (J->kind == 0) {
+        nw = stride_words(J->nvalues);
+        bm = (uint32_t *)malloc(nw * 4 + 4);
+        ids = (uint32_t *)malloc(sizeof(uint32_t) * ((uint64_t)J->nvalues + 1));
+    }
+    for (;;) {
+        uint64_t q = __sync_fetch_and_add(&J->next, 1);
+        if (q >= J->nq) break;
+        if (J->kind == 0) {
+            uint64_t c = or_union_csr(J->off, J->vals, J->nkeys, J->qkeys + J->qoff[q],
+                                      (uint32_t)(J->qoff[q + 1] - J->qoff[q]), J->nvalues, bm, ids);
+            J->counts[q] = c;
+            J->d0[q] = dg_bitmap(bm, nw);
+            J->d1[q] = dg_ids(ids, c, NULL);
+        } else if (J->kind == 1) {
+            J->d0[q] = dg_bitmap(J->data + q * J->stride, J->nwords);
+        } else {
+            J->d0[q] = dg_ids(J->data + J->doff[q], J->dcnt[q], J->asc ? &J->asc[q] : NULL);
+        }
+    }
+    free(bm);
+    free(ids);
+    return NULL;
+}
+
+static void run_digest(digest_job *J, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, digest_worker, J);
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    free(th);
+}
+
+void or_union_batch_digest(const uint64_t *off, const uint32_t *vals, uint32_t nkeys,
+                           const uint32_t *qkeys, const uint64_t *qoff, uint32_t nq,
+                           uint32_t nvalues, int nthreads, uint64_t *counts,
+                           uint64_t *bm_digest, uint64_t *id_digest)
+{
+    digest_job J;
+    memset(&J, 0, sizeof(J));
+    J.kind = 0; J.off = off; J.vals = vals; J.nkeys = nkeys; J.nvalues = nvalues;
+    J.qkeys = qkeys; J.qoff = qoff; J.nq = nq; J.counts = counts; J.d0 = bm_digest; J.d1 = id_digest;
+    run_digest(&J, nthreads);
+}
+
+void or_bitmap_digest(const uint32_t *bm, uint64_t stride_words, uint64_t nwords,
+                      uint64_t nq, int nthreads, uint64_t *out)
+{
+    digest_job J;
+    memset(&J, 0, sizeof(J));
+    J.kind = 1; J.data = bm; J.stride = stride_words; J.nwords = nwords; J.nq = nq; J.d0 = out;
+    run_digest(&J, nthreads);
+}
+
+void or_ids_digest(const uint32_t *ids, const uint64_t *off, const uint32_t *counts,
+                   uint64_t nq, int nthreads, uint64_t *out, uint8_t *ascending)
+{
+    digest_job J;
+    memset(&J, 0, sizeof(J));
+    J.kind = 2; J.data = ids; J.doff = off; J.dcnt = counts; J.nq = nq; J.d0 = out; J.asc = ascending;
+    run_digest(&J, nthreads);
+}
+
+/* ------------------------------------------------------------------------ */
 /* statistics                                                                */
 /* ------------------------------------------------------------------------ */
 

@@ -25,6 +25,10 @@
  *                          (gather + sort + dedupe vs. bitmap walk) and a
  *                          pure-Python set union on tiny ones, including
  *                          values at 2^32 - 2.
+ *   or_*_digest         -- or_union_batch_digest equals or_bitmap_digest /
+ *                          or_ids_digest of or_union_csr's own results, and
+ *                          a pure-Python evaluation of the two sums on
+ *                          small inputs; one flipped bit or id changes them.
  *   or_stats            -- closed forms (median of even/odd n, linear
  *                          interpolation) checked against numpy.percentile.
  *   or_mlp_*            -- finite differences and a closed-form 1-layer case.
@@ -126,6 +130,29 @@ double or_union_batch(const uint64_t *off, const uint32_t *vals, uint32_t nkeys,
                       const uint32_t *qkeys, const uint64_t *qoff, uint32_t nq,
                       uint32_t nvalues, int nthreads, uint64_t *counts,
                       double *lat_ms, uint64_t *checksum);
+
+/* Per-query digests, so that large batches can be compared query by query
+ * without holding every result twice.  A bitmap's digest is
+ *   sum over words w of bm[w] * wt(w)  (mod 2^64),  wt(w) = (w+1)*G | 1,
+ * G = 0x9E3779B97F4A7C15; an id list's digest is
+ *   sum over ids x of mix(x)  (mod 2^64),
+ *   mix(x) = x*0x9E3779B1 ^ (x ^ 0x5BD1E995)*0x27D4EB2F (64-bit products),
+ * and it is marked not ascending when some id is <= the one before it.
+ * (Checkers only: two results with equal count and digests are taken to be
+ * equal.)  or_union_batch_digest: the oracle's union of every query
+ * (or_union_csr) reduced to (count, bitmap digest, id digest), nthreads
+ * workers. */
+void or_union_batch_digest(const uint64_t *off, const uint32_t *vals, uint32_t nkeys,
+                           const uint32_t *qkeys, const uint64_t *qoff, uint32_t nq,
+                           uint32_t nvalues, int nthreads, uint64_t *counts,
+                           uint64_t *bm_digest, uint64_t *id_digest);
+/* Digests of nq bitmaps of nwords words at bm + q * stride_words. */
+void or_bitmap_digest(const uint32_t *bm, uint64_t stride_words, uint64_t nwords,
+                      uint64_t nq, int nthreads, uint64_t *out);
+/* Digests of nq id lists: query q's counts[q] ids at ids + off[q] (the
+ * offsets are relative to ids); ascending[q] = 1 when strictly ascending. */
+void or_ids_digest(const uint32_t *ids, const uint64_t *off, const uint32_t *counts,
+                   uint64_t nq, int nthreads, uint64_t *out, uint8_t *ascending);
 
 /* CalculateStats (test/adsfilter_q_all.cu:258-307): out = mean, median,
  * stddev (population), min, max, p50, p90, p95, p99. Sorts data in place. */

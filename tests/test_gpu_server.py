@@ -80,8 +80,9 @@ def test_packed_batch(c1, mode, aligned):
 
 def test_batch_beyond_grid_limit(c1):
     """70000 queries in one packed batch: more than a grid's y dimension holds
-    (65535), so the batch runs host-dispatched and its zero-fill, popcount and
-    id writer launch in slices of queries."""
+    (65535), so the per-query launches (zero-fill, group masks, popcount, id
+    writer) go in slices of queries, device-dispatched in graph mode and
+    host-dispatched in mode 0."""
     import torch
 
     ix, csr = c1
@@ -102,17 +103,21 @@ def test_batch_beyond_grid_limit(c1):
     dk = torch.from_numpy(qk.view(np.int32)).cuda()
     ids = torch.full((pos + 8,), -1, dtype=torch.int32, device="cuda")
     cnt = torch.empty(nq, dtype=torch.int32, device="cuda")
-    print(f"packed batch of {nq}", flush=True)
-    paths = ctx.query_batch_packed(dk, qo, ab.AB_PATH_THRESHOLD, ids, off, None, cnt)
-    ctx.sync()
-    assert (paths == ab.AB_PATH_DENSE).any() and (paths == ab.AB_PATH_SPARSE).any()
-    c = cnt.cpu().numpy().view(np.uint32)
-    assert np.array_equal(c.astype(np.int64), counts)
-    h = ids.cpu().numpy().view(np.uint32)
-    for i in list(range(0, nq, 97)) + list(range(0, nq, 5000)) + [65534, 65535, 65536, nq - 2, nq - 1]:
-        a, n = int(off[i]), int(counts[i])
-        assert np.array_equal(h[a:a + n], want[i]), f"q{i} ids"
-        assert (h[a + n:a + n + 4] == 0xFFFFFFFF).all(), f"q{i} wrote past its range"
+    for mode in (2, 0):
+        ctx.set_batch_mode(mode)
+        ids.fill_(-1)
+        cnt.fill_(-1)
+        print(f"packed batch of {nq}, mode {mode}", flush=True)
+        paths = ctx.query_batch_packed(dk, qo, ab.AB_PATH_THRESHOLD, ids, off, None, cnt)
+        ctx.sync()
+        assert (paths == ab.AB_PATH_DENSE).any() and (paths == ab.AB_PATH_SPARSE).any()
+        c = cnt.cpu().numpy().view(np.uint32)
+        assert np.array_equal(c.astype(np.int64), counts)
+        h = ids.cpu().numpy().view(np.uint32)
+        for i in list(range(0, nq, 97)) + list(range(0, nq, 5000)) + [65534, 65535, 65536, nq - 2, nq - 1]:
+            a, n = int(off[i]), int(counts[i])
+            assert np.array_equal(h[a:a + n], want[i]), f"mode {mode} q{i} ids"
+            assert (h[a + n:a + n + 4] == 0xFFFFFFFF).all(), f"mode {mode} q{i} wrote past its range"
     ctx.close()
 
 

@@ -297,3 +297,46 @@ def test_mlp_closed_forms():
     grad = np.array([0.3, -2.0, 1e-3, -5.0])
     oracle.adam_step(q, m, v, grad, 0.01, 0.9, 0.999, 1e-12, 1)
     assert np.allclose(q, np.arange(1.0, 5.0) - 0.01 * np.sign(grad), atol=1e-8)
+
+
+def _py_bitmap_digest(bm):
+    g = 0x9E3779B97F4A7C15
+    return sum(int(x) * ((((w + 1) * g) & (2**64 - 1)) | 1) for w, x in enumerate(bm)) % 2**64
+
+
+def _py_ids_digest(ids):
+    return sum((int(x) * 0x9E3779B1) ^ ((int(x) ^ 0x5BD1E995) * 0x27D4EB2F) for x in ids) % 2**64
+
+
+def test_digests_pinned():
+    """The per-query digests: the batch form equals the single-result forms on
+    or_union_csr's own outputs, both equal a pure-Python evaluation of their
+    definitions, and one changed bit or id changes them."""
+    rng = np.random.default_rng(21)
+    nk, nv = 300, 5000
+    k = rng.integers(0, nk, 20000).astype(np.uint32)
+    v = rng.integers(0, nv, 20000).astype(np.uint32)
+    csr = oracle.CSR(k, v, nk, nv)
+    qs = [rng.choice(nk + 20, size=int(rng.integers(0, 40)), replace=False).astype(np.uint32) for _ in range(25)]
+    qk = np.concatenate(qs)
+    qo = np.concatenate([[0], np.cumsum([q.size for q in qs])]).astype(np.uint64)
+    c, db, di = csr.union_batch_digest(qk, qo, nthreads=3)
+    bms, idl = zip(*[csr.union(q) for q in qs])
+    assert np.array_equal(c, [x.size for x in idl])
+    assert np.array_equal(db, oracle.bitmap_digest(np.stack(bms)))
+    flat = np.concatenate(idl)
+    offs = np.concatenate([[0], np.cumsum(c)[:-1]]).astype(np.uint64)
+    d2, asc = oracle.ids_digest(flat, offs, c, nthreads=2)
+    assert np.array_equal(di, d2) and asc.all()
+    for i in (0, 7, 24):
+        assert int(db[i]) == _py_bitmap_digest(bms[i]) and int(di[i]) == _py_ids_digest(idl[i])
+    j = int(np.argmax(c > 3))
+    b = bms[j].copy()
+    b[5] ^= 1 << 9
+    assert oracle.bitmap_digest(b)[0] != db[j]
+    x = idl[j].copy()
+    x[1] += 1 if x[1] + 1 < x[2] else 0
+    x[2] ^= 1
+    assert oracle.ids_digest(x, [0], [x.size])[0][0] != di[j]
+    y = idl[j][::-1].copy()
+    assert not oracle.ids_digest(y, [0], [y.size])[1][0]
