@@ -114,6 +114,7 @@ _SIGS = {
     "ab_query": (C.c_int, [_VP, _VP, C.c_uint32, C.c_int, _VP, _VP, _VP, C.POINTER(C.c_int)]),
     "ab_list_union_sparse": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint64]),
     "ab_list_union_dense": (C.c_int, [_VP, C.c_uint64, _VP, _VP, C.c_uint64]),
+    "ab_list_union_sparse_noalloc": (C.c_int, [_VP, _VP, C.c_uint32, _VP, C.c_uint64]),
     "ab_list_union_dense_batch": (C.c_int, [_VP, C.c_uint32, C.c_uint64, _VP, _VP, C.c_uint32]),
     "ab_list_union_sparse_batch": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_uint32, _VP, C.c_uint32]),
     "ab_query_batch": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, _VP, _VP, C.c_uint64, _VP, _VP]),

@@ -597,6 +597,12 @@ extern "C" ab_status ab_ctx_create(ab_index *ix, void *stream, ab_ctx **out)
     // clean query bitmap (1 bit per key) and query masks (u64 per key)
     AB_TRY(c->qbm.ensure(((uint64_t)ix->nkeys + 31) / 32 * 4 + 64));
     HIP_TRY(hipMemsetAsync(c->qbm.p, 0, c->qbm.cap, c->stream));
+    // set-up of ab_list_union_sparse_noalloc on this device; a failure is
+    // kept and reported by that entry point, not by context creation
+    {
+        const std::string keep = g_err;
+        if (noalloc_setup(ix->device) != AB_OK) g_err = keep;
+    }
     *out = c.release();
     return AB_OK;
 }

@@ -156,4 +156,7 @@ ab_status stage_wait(ab_ctx *c);
 ab_status mlp_decide(ab_ctx *c, int slot, const float *d_feat, uint32_t nq, uint8_t *d_sel);
 // Host decision for one query's features with model `slot`.
 int mlp_decide_host(const ab_ctx *c, int slot, const float *f);
+// Looks up the no-allocation entry point's per-device scratch address
+// (noalloc.hip); done once per device, at context creation.
+ab_status noalloc_setup(int dev);
 }  // namespace ab

@@ -253,6 +253,19 @@ ab_status ab_list_union_sparse(ab_ctx *ctx, const uint32_t *h_keys, uint32_t n,
 ab_status ab_list_union_dense(ab_ctx *ctx, uint64_t len_bytes, const uint8_t *h_bitmap,
                               uint32_t *d_res, uint64_t res_len_bytes);
 
+/* ab_list_union_sparse without any allocation at call time, for callers that
+ * cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.  Keys
+ * are device memory (d_keys, n <= 2^18 keys, else AB_INVALID_ARG); d_res
+ * holds at least ab_result_stride_bytes() bytes and receives the full
+ * stride.  The plan, scan and union take their scratch (5.25 MiB) from one
+ * fixed device array per GPU, held until the process ends; the call enqueues
+ * kernels on the context stream only (no allocation, no copy, no memset, no
+ * synchronization).  Calls that overlap on one GPU, from any contexts, share
+ * that array and race: serialize them (one stream, or events). */
+ab_status ab_list_union_sparse_noalloc(ab_ctx *ctx, const uint32_t *d_keys, uint32_t n,
+                                       uint32_t *d_res, uint64_t res_len_bytes);
+
 /* Batched dense method on B device bitmaps laid out query-major, each
  * len_bytes bytes (a multiple of 4); query i's result goes to
  * d_res + i * res_len_u32.  PrepareBatchDevice + RunArrays
