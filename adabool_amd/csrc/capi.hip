@@ -613,7 +613,8 @@ ab_ctx::~ab_ctx()
     for (DevBuf *b : {&qoff, &sel, &kq, &cnt, &koff, &parts, &feat, &M, &ccnt, &bm, &qlist, &keys, &qbm, &mask,
                       &mlp[0].params, &mlp[0].norm, &mlp[1].params, &mlp[1].norm, &qgroup, &grp_q, &grp_n, &plan,
                       &counter, &gmask, &ganybm, &idoff, &id_vals, &id_sorted, &id_ccnt, &id_cbase, &id_start,
-                      &id_tmp, &gkeys})
+                      &id_tmp, &gkeys, &sp_flag, &sp_cpre, &sp_scpre, &sp_rbase, &sp_slist, &sp_tot, &sp_refs,
+                      &sp_choff})
         b->release();
     for (PinBuf *b : {&hstage, &hkstage, &hsel, &hM})
         b->release();

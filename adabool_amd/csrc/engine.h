@@ -105,6 +105,11 @@ struct ab_ctx {
         std::vector<uint64_t> pending;  // the last shape run without a graph
     } gc;
     ab::DevBuf gkeys;  // graph mode: the batch's keys, copied in before every launch
+    // write-once sparse work of device-dispatched batches: per-query flags and
+    // prefixes, routed record refs (at most sp_cap) and per-chunk slice offsets
+    ab::DevBuf sp_flag, sp_cpre, sp_scpre, sp_rbase, sp_slist, sp_tot, sp_refs, sp_choff;
+    uint64_t sp_cap = 1ull << 28;
+    uint32_t sp_grid = 0;  // work-groups of the route launch
     uint32_t fused_grid = 0;  // work-groups of the last fused launch
     // per-kernel-class timing
     bool timing = false;

@@ -48,6 +48,8 @@ namespace {
 struct Plan {
     uint32_t nq;
     uint64_t K;
+    uint32_t nslices;  // value slices of a sparse query's bitmap
+    uint64_t sp_cap;   // routed-record budget (0: the atomic path for every sparse query)
     // graph mode: the keys are copied to ctx->gkeys before every launch and
     // the key count is read on the device, so one graph serves every batch
     // of the same query count whose key count rounds up to the same Kcap
@@ -95,29 +97,66 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
                              (uint32_t *)c->plan.p, s);
     }
     {
-        TimedSpan ts(c, AB_K_SPARSE_ZERO);
-        launch_zero_sparse(p.bm, p.stride, p.words, sel, nq, s);
-    }
-    {
         TimedSpan ts(c, AB_K_DENSE_MASK);
         launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
                            (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 0, s);
     }
     const uint64_t *Kd = p.graph ? qoff + nq : nullptr;  // qoff[nq] = the key count
+    SparseArgs sa{};
+    sa.keys = p.keys;
+    sa.kq = (const uint32_t *)c->kq.p;
+    sa.qoff = qoff;
+    sa.koff = (const uint64_t *)c->koff.p;
+    sa.K = p.K;
+    sa.Kd = Kd;
+    sa.nq = nq;
+    sa.chunk = c->sparse_chunk;
+    sa.nslices = p.nslices;
+    sa.flag = (uint8_t *)c->sp_flag.p;
+    sa.cpre = (uint32_t *)c->sp_cpre.p;
+    sa.scpre = (uint32_t *)c->sp_scpre.p;
+    sa.rbase = (uint64_t *)c->sp_rbase.p;
+    sa.slist = (uint32_t *)c->sp_slist.p;
+    sa.tot = (uint32_t *)c->sp_tot.p;
+    sa.refs = (uint32_t *)c->sp_refs.p;
+    sa.choff = (uint16_t *)c->sp_choff.p;
+    sa.cap = p.sp_cap;
+    sa.out = p.bm;
+    sa.out_stride = p.stride;
     if (p.Kcap) {
-        TimedSpan ts(c, AB_K_SPARSE_PLAN);
-        launch_sparse_plan(d, p.keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
-        launch_scan_u64((const uint64_t *)c->cnt.p, (uint64_t *)c->koff.p, p.Kcap, (uint64_t *)c->parts.p, s, Kd);
+        {
+            TimedSpan ts(c, AB_K_SPARSE_PLAN);
+            launch_sparse_plan(d, p.keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
+            launch_scan_u64((const uint64_t *)c->cnt.p, (uint64_t *)c->koff.p, p.Kcap, (uint64_t *)c->parts.p, s,
+                            Kd);
+            launch_sparse_split(sel, sa, s);
+        }
+        {
+            // only the atomic path needs zeroed bitmaps; slice queries write theirs whole
+            TimedSpan ts(c, AB_K_SPARSE_ZERO);
+            launch_zero_sparse(p.bm, p.stride, p.words, sa.flag, nq, kSpAtomic, s);
+        }
+        TimedSpan ts(c, AB_K_SPARSE_UNION);
+        HIP_TRY(launch_sparse_route(d, sa, c->sp_grid, s));
+    } else {
+        // no keys at all: every sparse query's result is empty
+        TimedSpan ts(c, AB_K_SPARSE_ZERO);
+        launch_zero_sparse(p.bm, p.stride, p.words, sel, nq, AB_PATH_SPARSE, s);
     }
     {
         TimedSpan ts(c, AB_K_FUSED_UNION);
         FusedArgs f{};
-        f.keys = p.keys;
-        f.kq = (const uint32_t *)c->kq.p;
-        f.koff = (const uint64_t *)c->koff.p;
-        f.K = p.Kcap ? p.K : 0;
-        f.Kd = p.Kcap ? Kd : nullptr;
-        f.chunk = c->sparse_chunk;
+        if (p.Kcap) {
+            f.sp_tot = sa.tot;
+            f.sp_slist = sa.slist;
+            f.sp_cpre = sa.cpre;
+            f.sp_scpre = sa.scpre;
+            f.sp_refs = sa.refs;
+            f.sp_rbase = sa.rbase;
+            f.sp_choff = sa.choff;
+            f.sp_chunk = sa.chunk;
+            f.nslices = sa.nslices;
+        }
         f.plan = (const uint32_t *)c->plan.p;
         f.grp_q = (const uint32_t *)c->grp_q.p;
         f.grp_n = (const uint32_t *)c->grp_n.p;
@@ -182,12 +221,21 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     p.any_stride = (nkeys + 31) / 32 + 16;
     const size_t esize = p.mode == kModeMask32 ? 4 : (p.mode == kModeMask64 ? 8 : (p.mode == kModeMask128 ? 16 : 32));
 
+    // write-once sparse: slices of 2^18 values; refs hold type << 29 | index
+    p.nslices = (uint32_t)std::max<uint64_t>(1, ((uint64_t)ix->nvalues + (1ull << kSpSliceShift) - 1) >> kSpSliceShift);
+    p.sp_cap = c->sp_cap;
+    for (int t = 0; t < AB_NTYPES; ++t)
+        if (ix->nrec[t] >= (1ull << kSpTypeShift)) p.sp_cap = 0;
+    if (p.nslices > kSpMaxSlices) p.sp_cap = 0;
+
     // ---- every buffer exists before anything is enqueued (capture-safe) ----
     const uint64_t nchunk = (p.words + kCompactWords - 1) / kCompactWords;
+    std::vector<DevBuf *> bufs = {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan,
+                                  &c->counter, &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm, &c->gkeys,
+                                  &c->sp_flag, &c->sp_cpre, &c->sp_scpre, &c->sp_rbase, &c->sp_slist, &c->sp_tot,
+                                  &c->sp_refs, &c->sp_choff};
     std::vector<void *> before;
-    for (DevBuf *b : {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan, &c->counter,
-                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm, &c->gkeys})
-        before.push_back(b->p);
+    for (DevBuf *b : bufs) before.push_back(b->p);
     p.packed = a.h_id_off && a.d_ids;
     p.stage_bytes = ((size_t)nq + 1) * 8 + (p.packed ? (size_t)nq * 8 : 0);
     AB_TRY(c->qoff.ensure(p.stage_bytes));
@@ -205,6 +253,27 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     AB_TRY(c->parts.ensure(scan_parts_needed(p.Kcap) * 8 + 16));
     if (p.graph) AB_TRY(c->gkeys.ensure(p.Kcap * 4 + 16));
     AB_TRY(c->ccnt.ensure((size_t)nq * nchunk * 4 + 4));
+    if (p.Kcap) {
+        AB_TRY(c->sp_flag.ensure(nq + 16));
+        AB_TRY(c->sp_cpre.ensure(((size_t)nq + 1) * 4));
+        AB_TRY(c->sp_scpre.ensure((size_t)nq * 4 + 4));
+        AB_TRY(c->sp_rbase.ensure((size_t)nq * 8 + 8));
+        AB_TRY(c->sp_slist.ensure((size_t)nq * 4 + 4));
+        AB_TRY(c->sp_tot.ensure(16));
+        if (p.sp_cap) {
+            // a slice query of M records has ceil(M / chunk) <= M / chunk + 1 chunks
+            const uint64_t max_chunks = p.sp_cap / c->sparse_chunk + nq + 1;
+            AB_TRY(c->sp_refs.ensure(p.sp_cap * 4));
+            AB_TRY(c->sp_choff.ensure(max_chunks * (p.nslices + 1) * 2));
+        }
+        if (!c->sp_grid) {
+            int dev = 0, cus = 0, per = 0;
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            per = 4;
+            c->sp_grid = (uint32_t)(cus * per);
+        }
+    }
     p.bm = a.d_bitmaps;
     if (!p.bm) {
         AB_TRY(c->bm.ensure((size_t)nq * p.stride * 4));
@@ -223,9 +292,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
         release_graph(c);
     }
     std::vector<void *> after;
-    for (DevBuf *b : {&c->qoff, &c->sel, &c->feat, &c->M, &c->qgroup, &c->grp_q, &c->grp_n, &c->plan, &c->counter,
-                      &c->kq, &c->cnt, &c->koff, &c->parts, &c->ccnt, &c->bm, &c->gkeys})
-        after.push_back(b->p);
+    for (DevBuf *b : bufs) after.push_back(b->p);
     if (before != after) release_graph(c);
     p.keys = p.graph ? (const uint32_t *)c->gkeys.p : a.d_keys;
 
@@ -240,7 +307,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     } else {
         // this batch's keys into the buffer the graph reads
         if (p.K) HIP_TRY(hipMemcpyAsync(c->gkeys.p, a.d_keys, p.K * 4, hipMemcpyDeviceToDevice, s));
-        std::vector<uint64_t> key = {nq, p.Kcap, (uint64_t)a.path, p.grid,
+        std::vector<uint64_t> key = {nq, p.Kcap, (uint64_t)a.path, p.grid, p.sp_cap, c->sparse_chunk,
                                      (uint64_t)(uintptr_t)p.bm, p.stride, p.words, (uint64_t)(uintptr_t)a.d_ids,
                                      a.ids_stride, (uint64_t)p.packed, (uint64_t)a.id_off_aligned,
                                      (uint64_t)(uintptr_t)a.d_counts, c->cfg_gen,

@@ -66,18 +66,55 @@ hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, co
 // Device-dispatched batch: group assignment, sparse zero-fill, group masks.
 void launch_assign_groups(const uint8_t *sel, uint32_t nq, uint32_t G, uint32_t *qgroup, uint32_t *grp_q,
                           uint32_t *grp_n, uint32_t *plan, hipStream_t s);
+// Zeroes the bitmaps of the queries with sel[q] == match.
 void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq,
-                        hipStream_t s);
+                        uint8_t match, hipStream_t s);
 void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, const uint32_t *qgroup,
                         uint32_t nq, void *masks, uint64_t mask_stride, uint32_t *anybms,
                         uint64_t any_stride, uint32_t nkeys, int clear, hipStream_t s);
+// ---- write-once sparse work of device-dispatched batches ----------------
+// A sparse query's bitmap is cut into value slices of 2^kSpSliceShift values
+// (32 KB of bits); a route pass sorts the query's records by slice, and one
+// work item per (query, slice) ORs that slice's records into LDS and writes
+// the slice once.  Queries beyond the context's record budget, and indices
+// with more slices or records than the references encode, take the atomic
+// path instead (zero-fill, then one global atomic per touched word).
+constexpr uint32_t kSpSliceShift = 18;
+constexpr uint32_t kSpSliceWords = 1u << (kSpSliceShift - 5);
+constexpr uint32_t kSpMaxSlices = 1024;
+constexpr uint32_t kSpTypeShift = 29;  // ref = type << 29 | record index (< 2^29)
+enum SparseFlag : uint8_t { kSpNone = 0, kSpAtomic = 1, kSpSlice = 2 };
+
+// Arguments of the split, route and emit stages.
+struct SparseArgs {
+    const uint32_t *keys, *kq;  // keys, query of each key
+    const uint64_t *qoff;       // nq+1 key offsets per query
+    const uint64_t *koff;       // record prefix over keys (K+1)
+    uint64_t K;
+    const uint64_t *Kd;         // when set, K is read here on the device
+    uint32_t nq, chunk;         // queries; records per route chunk (<= 65535)
+    uint32_t nslices;           // value slices per bitmap
+    uint8_t *flag;              // per query: kSpNone (dense) / kSpAtomic / kSpSlice
+    uint32_t *cpre;             // nq+1: route-chunk prefix over queries
+    uint32_t *scpre;            // nq: slice-chunk prefix (slice queries)
+    uint64_t *rbase;            // nq: first ref of a slice query
+    uint32_t *slist;            // slice queries in order
+    uint32_t *tot;              // [0] route chunks, [1] slice queries, [2] slice chunks
+    uint32_t *refs;             // routed records, sorted by slice within each chunk
+    uint16_t *choff;            // per slice chunk: nslices+1 offsets into its refs
+    uint64_t cap;               // refs capacity (0: every sparse query atomic)
+    uint32_t *out;
+    uint64_t out_stride;
+};
+// One work-group: flags, prefixes and totals of the sparse queries.
+void launch_sparse_split(const uint8_t *sel, const SparseArgs &a, hipStream_t s);
+// Route chunks: slice queries' records sorted into refs; atomic queries'
+// records OR-ed into their (zeroed) bitmaps.  Persistent over `grid`.
+hipError_t launch_sparse_route(const DevIndex &ix, const SparseArgs &a, uint32_t grid, hipStream_t s);
+uint32_t sparse_route_lds_bytes(uint32_t chunk);
+
 // Arguments of the persistent union kernel.
 struct FusedArgs {
-    const uint32_t *keys, *kq;   // sparse: keys, query of each key
-    const uint64_t *koff;        // sparse: record prefix over keys (K+1)
-    uint64_t K;
-    const uint64_t *Kd;          // when set, K is read here on the device
-    uint32_t chunk;              // sparse items per work item
     const uint32_t *plan;        // [0] dense groups
     const uint32_t *grp_q;       // 64 query slots per group
     const uint32_t *grp_n;       // queries per group
@@ -89,6 +126,11 @@ struct FusedArgs {
     uint64_t out_stride, out_words;
     int use_col;
     uint32_t *counter;           // work counter (reset by the launcher)
+    // write-once sparse: one emit item per (slice query, slice), after the dense items
+    const uint32_t *sp_tot, *sp_slist, *sp_cpre, *sp_scpre, *sp_refs;
+    const uint64_t *sp_rbase;
+    const uint16_t *sp_choff;
+    uint32_t sp_chunk, nslices;
 };
 uint32_t fused_lds_bytes(int mode, uint32_t G, uint32_t tile_shift, uint32_t threads, bool use_col);
 // Work-groups of the persistent union launch: CUs x resident work-groups per

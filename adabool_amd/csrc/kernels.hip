@@ -9,6 +9,8 @@
 // is OR-ed once.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "adabool.h"
 #include "device.h"
 #include "kernels.h"
@@ -215,15 +217,16 @@ __device__ __forceinline__ void emit_global(const DevIndex &ix, uint32_t idx, ui
 // into <=256-record units, src/AdaBool/load_balance_transformer.h:50-69).
 constexpr uint32_t kSparseNT = 256, kSparseWin = 1024;
 
-// Items [c0, c1) of the sparse work: every thread of the work-group takes
+// Records [c0, c1) of the sparse work: every thread of the work-group takes
 // items c0 + tid, c0 + tid + NT, ...; skoff is an LDS window of kSparseWin+1
-// key offsets, s_lo/s_hi LDS scratch of the search.
-template <int NT>
-__device__ __forceinline__ void sparse_chunk(const DevIndex &ix, const uint32_t *__restrict__ keys,
-                                             const uint32_t *__restrict__ kq, const uint64_t *__restrict__ koff,
-                                             uint64_t K, uint64_t c0, uint64_t c1, uint32_t *__restrict__ out,
-                                             uint64_t out_stride, unsigned long long *skoff,
-                                             unsigned long long *s_lo, unsigned long long *s_hi)
+// key offsets, s_lo/s_hi LDS scratch of the search.  Calls
+// f(std::integral_constant<int, T>, idx, it, q) for record idx of type T (the
+// key-major arrays), item number it, of query q.
+template <int NT, class F>
+__device__ __forceinline__ void sparse_records(const DevIndex &ix, const uint32_t *__restrict__ keys,
+                                               const uint32_t *__restrict__ kq, const uint64_t *__restrict__ koff,
+                                               uint64_t K, uint64_t c0, uint64_t c1, unsigned long long *skoff,
+                                               unsigned long long *s_lo, unsigned long long *s_hi, F &&f)
 {
     uint64_t jw = wg_find<NT>(koff, K, c0, s_lo, s_hi);
     for (;;) {
@@ -245,26 +248,40 @@ __device__ __forceinline__ void sparse_chunk(const DevIndex &ix, const uint32_t 
             const uint32_t q = kq[j];
             const uint4 *rows = reinterpret_cast<const uint4 *>(ix.seg8 + (uint64_t)k * 8);
             const uint4 a0 = rows[0], a1 = rows[1], b0 = rows[2], b1 = rows[3];
-            uint32_t *bm = out + (uint64_t)q * out_stride;
             const uint32_t l0 = b0.x - a0.x, l1 = b0.y - a0.y, l2 = b0.z - a0.z, l3 = b0.w - a0.w,
                            l4 = b1.x - a1.x;
-            if (r < l0) { emit_global<AB_ARRAY1>(ix, a0.x + r, bm); continue; }
+            if (r < l0) { f(std::integral_constant<int, AB_ARRAY1>{}, a0.x + r, it, q); continue; }
             r -= l0;
-            if (r < l1) { emit_global<AB_ARRAY4>(ix, a0.y + r, bm); continue; }
+            if (r < l1) { f(std::integral_constant<int, AB_ARRAY4>{}, a0.y + r, it, q); continue; }
             r -= l1;
-            if (r < l2) { emit_global<AB_ARRAY8>(ix, a0.z + r, bm); continue; }
+            if (r < l2) { f(std::integral_constant<int, AB_ARRAY8>{}, a0.z + r, it, q); continue; }
             r -= l2;
-            if (r < l3) { emit_global<AB_ARRAY16>(ix, a0.w + r, bm); continue; }
+            if (r < l3) { f(std::integral_constant<int, AB_ARRAY16>{}, a0.w + r, it, q); continue; }
             r -= l3;
-            if (r < l4) { emit_global<AB_ARRAY32>(ix, a1.x + r, bm); continue; }
+            if (r < l4) { f(std::integral_constant<int, AB_ARRAY32>{}, a1.x + r, it, q); continue; }
             r -= l4;
-            emit_global<AB_BITSET>(ix, a1.y + r, bm);
+            f(std::integral_constant<int, AB_BITSET>{}, a1.y + r, it, q);
         }
         const bool done = (skoff[nwin] >= c1) || jn >= K;
         __syncthreads();
         if (done) break;
         jw = jn;
     }
+}
+
+// The records [c0, c1), each OR-ed into its query's bitmap with one global
+// atomic per touched word (the bitmaps are zeroed beforehand).
+template <int NT>
+__device__ __forceinline__ void sparse_chunk(const DevIndex &ix, const uint32_t *__restrict__ keys,
+                                             const uint32_t *__restrict__ kq, const uint64_t *__restrict__ koff,
+                                             uint64_t K, uint64_t c0, uint64_t c1, uint32_t *__restrict__ out,
+                                             uint64_t out_stride, unsigned long long *skoff,
+                                             unsigned long long *s_lo, unsigned long long *s_hi)
+{
+    sparse_records<NT>(ix, keys, kq, koff, K, c0, c1, skoff, s_lo, s_hi,
+                       [&](auto t, uint32_t idx, uint64_t, uint32_t q) {
+                           emit_global<decltype(t)::value>(ix, idx, out + (uint64_t)q * out_stride);
+                       });
 }
 
 __global__ __launch_bounds__(kSparseNT) void k_sparse_union(DevIndex ix, const uint32_t *__restrict__ keys,
@@ -889,10 +906,10 @@ void launch_assign_groups(const uint8_t *sel, uint32_t nq, uint32_t G, uint32_t 
 }
 
 __global__ __launch_bounds__(256) void k_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words,
-                                                     const uint8_t *__restrict__ sel)
+                                                     const uint8_t *__restrict__ sel, uint8_t match)
 {
     const uint32_t q = blockIdx.y;
-    if (sel[q] != AB_PATH_SPARSE) return;
+    if (sel[q] != match) return;
     const uint64_t w0 = (uint64_t)blockIdx.x * kZeroChunk;
     const uint64_t w1 = w0 + kZeroChunk < words ? w0 + kZeroChunk : words;
     uint32_t *dst = bm + (uint64_t)q * stride;
@@ -902,14 +919,15 @@ __global__ __launch_bounds__(256) void k_zero_sparse(uint32_t *bm, uint64_t stri
 // queries go on the grid's y dimension, in slices of at most 65535
 constexpr uint32_t kMaxGridY = 65535;
 
-void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq, hipStream_t s)
+void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq,
+                        uint8_t match, hipStream_t s)
 {
     if (!nq || !words) return;
     const uint32_t gx = (uint32_t)((words + kZeroChunk - 1) / kZeroChunk);
     for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
         const uint32_t m = nq - q0 < kMaxGridY ? nq - q0 : kMaxGridY;
         hipLaunchKernelGGL(k_zero_sparse, dim3(gx, m), dim3(256), 0, s, bm + (uint64_t)q0 * stride, stride, words,
-                           sel + q0);
+                           sel + q0, match);
     }
 }
 
@@ -966,26 +984,217 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// write-once sparse: split, route, and the per-(query, slice) emit
+// ---------------------------------------------------------------------------
+// One work-group of 1024 threads, queries in order.  A sparse query gets
+// ceil(M / chunk) route chunks (M = its records, from the key prefix koff);
+// while the running record total of sparse queries stays within cap it takes
+// the slice path (its refs start at that running total), and every later one
+// the atomic path.  Totals: tot[0] route chunks, tot[1] slice queries,
+// tot[2] slice chunks.
+__global__ __launch_bounds__(1024) void k_sparse_split(const uint8_t *__restrict__ sel, SparseArgs a)
+{
+    __shared__ unsigned long long sh[16];
+    const uint64_t K = a.Kd ? *a.Kd : a.K;
+    unsigned long long run_ch = 0, run_m = 0, run_sq = 0, run_sch = 0;
+    for (uint32_t base = 0; base < a.nq; base += 1024) {
+        const uint32_t q = base + threadIdx.x;
+        const bool in = q < a.nq;
+        const bool sparse = in && sel[q] == AB_PATH_SPARSE;
+        uint64_t m = 0;
+        if (sparse && K) {
+            const uint64_t j0 = a.qoff[q], j1 = a.qoff[q + 1];
+            m = a.koff[j1] - a.koff[j0];
+        }
+        const unsigned long long nch = sparse ? (m + a.chunk - 1) / a.chunk : 0;
+        unsigned long long tot;
+        const unsigned long long ch_before = run_ch + block_excl_scan(nch, sh, &tot);
+        run_ch += tot;
+        const unsigned long long m_before = run_m + block_excl_scan(sparse ? m : 0, sh, &tot);
+        run_m += tot;
+        const bool slice = sparse && a.cap && m_before + m <= a.cap;
+        const unsigned long long rank = run_sq + block_excl_scan(slice ? 1ull : 0ull, sh, &tot);
+        run_sq += tot;
+        const unsigned long long sch_before = run_sch + block_excl_scan(slice ? nch : 0ull, sh, &tot);
+        run_sch += tot;
+        if (in) {
+            a.flag[q] = sparse ? (slice ? kSpSlice : kSpAtomic) : kSpNone;
+            a.cpre[q] = (uint32_t)ch_before;
+            if (slice) {
+                a.scpre[q] = (uint32_t)sch_before;
+                a.rbase[q] = m_before;
+                a.slist[rank] = q;
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        a.cpre[a.nq] = (uint32_t)run_ch;
+        a.tot[0] = (uint32_t)run_ch;
+        a.tot[1] = (uint32_t)run_sq;
+        a.tot[2] = (uint32_t)run_sch;
+    }
+}
+
+void launch_sparse_split(const uint8_t *sel, const SparseArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sparse_split, dim3(1), dim3(1024), 0, s, sel, a);
+}
+
+uint32_t sparse_route_lds_bytes(uint32_t chunk)
+{
+    // key-offset window | slice histogram | staged refs | staged slices
+    return (kSparseWin + 1) * 8 + (kSpMaxSlices + 1) * 4 + chunk * 4 + ((chunk * 2 + 15) & ~15u);
+}
+
+// Route chunk r: query q = the last with cpre[q] <= r, records
+// [qb + c * chunk, ...) of its key-prefix range.  Atomic queries: OR-ed into
+// the bitmap.  Slice queries: each record's ref staged in LDS with its slice,
+// counted per slice, the chunk's slice offsets written to choff, and the refs
+// written slice by slice to refs[rbase[q] + c * chunk ...].
+template <int NT>
+__global__ __launch_bounds__(NT) void k_sparse_route(DevIndex ix, SparseArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    __shared__ unsigned long long s_lo, s_hi, sh[NT / 64];
+    __shared__ uint32_t s_q;
+    unsigned long long *skoff = reinterpret_cast<unsigned long long *>(lds);
+    uint32_t *hist = lds + 2 * (kSparseWin + 1);
+    uint32_t *st_ref = hist + (kSpMaxSlices + 1);
+    uint16_t *st_sl = reinterpret_cast<uint16_t *>(st_ref + a.chunk);
+    const uint64_t K = a.Kd ? *a.Kd : a.K;
+    const uint32_t nr = a.tot[0];
+    const uint32_t S = a.nslices;
+    for (uint32_t r = blockIdx.x; r < nr; r += gridDim.x) {
+        if (threadIdx.x == 0) {
+            uint32_t lo = 0, hi = a.nq;  // cpre[lo] <= r < cpre[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.cpre[mid] <= r) lo = mid; else hi = mid;
+            }
+            s_q = lo;
+        }
+        __syncthreads();
+        const uint32_t q = s_q;
+        const uint32_t c = r - a.cpre[q];
+        const uint64_t qb = a.koff[a.qoff[q]], qe = a.koff[a.qoff[q + 1]];
+        const uint64_t c0 = qb + (uint64_t)c * a.chunk;
+        const uint64_t c1 = c0 + a.chunk < qe ? c0 + a.chunk : qe;
+        if (a.flag[q] != kSpSlice) {
+            sparse_chunk<NT>(ix, a.keys, a.kq, a.koff, K, c0, c1, a.out, a.out_stride, skoff, &s_lo, &s_hi);
+            continue;  // sparse_chunk ends with a barrier
+        }
+        for (uint32_t i = threadIdx.x; i <= S; i += NT) hist[i] = 0;
+        __syncthreads();
+        sparse_records<NT>(ix, a.keys, a.kq, a.koff, K, c0, c1, skoff, &s_lo, &s_hi,
+                           [&](auto t, uint32_t idx, uint64_t it, uint32_t) {
+                               constexpr int T = decltype(t)::value;
+                               const uint32_t sl = ix.hdr[T][idx] >> kSpSliceShift;
+                               const uint32_t i = (uint32_t)(it - c0);
+                               st_ref[i] = ((uint32_t)T << kSpTypeShift) | idx;
+                               st_sl[i] = (uint16_t)sl;
+                               atomicAdd(&hist[sl], 1u);
+                           });
+        // (sparse_records ends with a barrier)  per-slice offsets, in place
+        const uint32_t per = (S + NT - 1) / NT, b0 = threadIdx.x * per;
+        unsigned long long mine = 0;
+        for (uint32_t b = b0; b < b0 + per && b < S; ++b) mine += hist[b];
+        unsigned long long total;
+        unsigned long long run = block_excl_scan(mine, sh, &total);
+        uint16_t *co = a.choff + (uint64_t)(a.scpre[q] + c) * (S + 1);
+        for (uint32_t b = b0; b < b0 + per && b < S; ++b) {
+            const uint32_t v = hist[b];
+            hist[b] = (uint32_t)run;
+            co[b] = (uint16_t)run;
+            run += v;
+        }
+        const uint32_t n = (uint32_t)(c1 - c0);
+        if (threadIdx.x == 0) co[S] = (uint16_t)n;
+        __syncthreads();
+        uint32_t *dst = a.refs + a.rbase[q] + (uint64_t)c * a.chunk;
+        for (uint32_t i = threadIdx.x; i < n; i += NT) {
+            const uint32_t pos = atomicAdd(&hist[st_sl[i]], 1u);
+            dst[pos] = st_ref[i];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_sparse_route(const DevIndex &ix, const SparseArgs &a, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sparse_route<kSparseNT>, dim3(grid), dim3(kSparseNT), sparse_route_lds_bytes(a.chunk), s,
+                       ix, a);
+    return hipGetLastError();
+}
+
+// ORs record `ref` (type << 29 | index, key-major arrays) into an LDS slice
+// whose first word is w0.
+__device__ __forceinline__ void emit_ref_lds(const DevIndex &ix, uint32_t ref, uint32_t *slice, uint32_t w0)
+{
+    const uint32_t idx = ref & ((1u << kSpTypeShift) - 1);
+    auto put = [&](uint32_t word, uint32_t bits) { atomicOr(&slice[word - w0], bits); };
+    uint32_t w[8];
+    switch (ref >> kSpTypeShift) {
+    case AB_ARRAY1: {
+        const uint32_t h = ix.hdr[AB_ARRAY1][idx];
+        put(h >> 5, 1u << (h & 31));
+        break;
+    }
+    case AB_ARRAY4: load_payload<AB_ARRAY4>(ix.pay[AB_ARRAY4], idx, w); emit_record<AB_ARRAY4>(ix.hdr[AB_ARRAY4][idx], w, put); break;
+    case AB_ARRAY8: load_payload<AB_ARRAY8>(ix.pay[AB_ARRAY8], idx, w); emit_record<AB_ARRAY8>(ix.hdr[AB_ARRAY8][idx], w, put); break;
+    case AB_ARRAY16: load_payload<AB_ARRAY16>(ix.pay[AB_ARRAY16], idx, w); emit_record<AB_ARRAY16>(ix.hdr[AB_ARRAY16][idx], w, put); break;
+    case AB_ARRAY32: load_payload<AB_ARRAY32>(ix.pay[AB_ARRAY32], idx, w); emit_record<AB_ARRAY32>(ix.hdr[AB_ARRAY32][idx], w, put); break;
+    default: load_payload<AB_BITSET>(ix.pay[AB_BITSET], idx, w); emit_record<AB_BITSET>(ix.hdr[AB_BITSET][idx], w, put); break;
+    }
+}
+
+// Slice s of slice query q: its records from every route chunk of q, OR-ed
+// into LDS, then every word of the slice written once.
+template <int NT>
+__device__ __forceinline__ void sparse_emit_slice(const DevIndex &ix, const FusedArgs &a, uint32_t q, uint32_t s,
+                                                  uint32_t *slice)
+{
+    const uint64_t w0 = (uint64_t)s * kSpSliceWords;
+    for (uint32_t i = threadIdx.x; i < kSpSliceWords; i += NT) slice[i] = 0;
+    __syncthreads();
+    const uint32_t S = a.nslices;
+    const uint32_t nch = a.sp_cpre[q + 1] - a.sp_cpre[q];
+    const uint16_t *co = a.sp_choff + (uint64_t)a.sp_scpre[q] * (S + 1);
+    const uint32_t *refs = a.sp_refs + a.sp_rbase[q];
+    for (uint32_t j = 0; j < nch; ++j, co += S + 1, refs += a.sp_chunk) {
+        const uint32_t lo = co[s], hi = co[s + 1];
+        for (uint32_t t = lo + threadIdx.x; t < hi; t += NT) emit_ref_lds(ix, refs[t], slice, (uint32_t)w0);
+    }
+    __syncthreads();
+    if (w0 >= a.out_words) return;  // block-uniform
+    const uint64_t nw = a.out_words - w0 < kSpSliceWords ? a.out_words - w0 : kSpSliceWords;
+    uint32_t *dst = a.out + (uint64_t)q * a.out_stride + w0;
+    if ((a.out_stride & 3) == 0 && (nw & 3) == 0) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(slice);
+        for (uint32_t i = threadIdx.x; i < nw / 4; i += NT) reinterpret_cast<uint4 *>(dst)[i] = src[i];
+    } else {
+        for (uint32_t i = threadIdx.x; i < nw; i += NT) dst[i] = slice[i];
+    }
+}
+
 __global__ void k_set_u32(uint32_t *p, uint32_t v)
 {
     if (threadIdx.x == 0) *p = v;
 }
 
 // Persistent: every work-group claims items from one counter -- first the
-// dense (group, tile) items, tile by tile, then the sparse chunks -- until
-// none are left.  Dense items come first: they are the long ones.
+// dense (group, tile) items, tile by tile, then the (slice query, slice)
+// emit items of the write-once sparse work -- until none are left.  Dense
+// items come first: they are the long ones.
 template <int MODE, int NT>
 __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
 {
     using M = typename MaskOf<MODE>::T;
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
     __shared__ uint32_t s_item;
-    __shared__ unsigned long long s_lo, s_hi;
     const uint32_t ngroups = a.plan[0];
     const uint64_t nd = (uint64_t)ngroups * ix.ntiles;
-    const uint64_t K = a.Kd ? *a.Kd : a.K;
-    const uint64_t W = K ? a.koff[K] : 0;
-    const uint64_t ns = (W + a.chunk - 1) / a.chunk;
+    const uint64_t ne = a.sp_tot ? (uint64_t)a.sp_tot[1] * a.nslices : 0;
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(a.counter, 1u);
         __syncthreads();
@@ -998,10 +1207,10 @@ __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
             dense_tile<MODE, NT>(ix, tile, reinterpret_cast<const M *>(a.masks) + (uint64_t)g * a.mask_stride,
                                  a.anybms + (uint64_t)g * a.any_stride, a.grp_n[g], a.grp_q + (uint64_t)g * kGroupSlots,
                                  a.out, a.out_stride, a.out_words, a.use_col, lds);
-        } else if (item < nd + ns) {
-            const uint64_t c0 = (item - nd) * a.chunk, c1 = (c0 + a.chunk < W) ? c0 + a.chunk : W;
-            sparse_chunk<NT>(ix, a.keys, a.kq, a.koff, K, c0, c1, a.out, a.out_stride,
-                             reinterpret_cast<unsigned long long *>(lds), &s_lo, &s_hi);
+        } else if (item < nd + ne) {
+            const uint32_t e = (uint32_t)(item - nd);
+            const uint32_t qi = e / a.nslices, sl = e - qi * a.nslices;
+            sparse_emit_slice<NT>(ix, a, a.sp_slist[qi], sl, lds);
         } else {
             break;
         }
@@ -1012,7 +1221,7 @@ __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
 uint32_t fused_lds_bytes(int mode, uint32_t G, uint32_t tile_shift, uint32_t threads, bool use_col)
 {
     const uint32_t d = dense_lds_bytes(mode, G, tile_shift, threads, use_col);
-    const uint32_t sp = (kSparseWin + 1) * 8;
+    const uint32_t sp = kSpSliceWords * 4;  // one value slice of a sparse query
     return d > sp ? d : sp;
 }
 
