@@ -10,9 +10,9 @@ each is timed.  Two label sets, one per model slot:
   times of device-dispatched batches (one fused union launch) over random
   groups of 128 queries, each group run once on either path: dense -- the
   group's mask and union kernels divided by the group size (one pass over the
-  index serves the group); sparse -- the zero-fill and plan divided by the
-  group size, plus the union in proportion to the query's records M(q) (the
-  load-balanced union's work).  Timing a sparse query alone would charge it
+  index serves the group); sparse -- the plan divided by the group size, plus
+  the route pass and the slice items in proportion to the query's records
+  M(q) (the work of both follows the records).  Timing a sparse query alone would charge it
   a whole launch that a batch shares.
 
 Each model (12 features -> 64 -> 64 -> 2, log milliseconds of dense and
@@ -134,7 +134,9 @@ def time_batch_shares(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarra
                 fixed = (k["sparse_zero"][0] + k["sparse_plan"][0]) / members.size
                 w = M[members].astype(np.float64)
                 frac = w / w.sum() if w.sum() > 0 else np.full(members.size, 1.0 / members.size)
-                t[members, 1] = np.minimum(t[members, 1], fixed + k["fused_union"][0] * frac)
+                # the route pass and the fused launch's slice items follow the records
+                work = k["sparse_union"][0] + k["fused_union"][0]
+                t[members, 1] = np.minimum(t[members, 1], fixed + work * frac)
     finally:
         ctx.set_timing(False)
     # a query with no records costs its zero-fill only; keep labels positive

@@ -79,28 +79,6 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     uint8_t *sel = (uint8_t *)c->sel.p;
     // qoff and, for packed ids, the id offsets arrive in one copy
     HIP_TRY(hipMemcpyAsync(c->qoff.p, c->hstage.p, p.stage_bytes, hipMemcpyHostToDevice, s));
-    {
-        TimedSpan ts(c, AB_K_DECIDE);
-        if (a.path == AB_PATH_DENSE || a.path == AB_PATH_SPARSE) {
-            hipLaunchKernelGGL(k_fill_u8, dim3((nq + 255) / 256), dim3(256), 0, s, sel, nq, (uint8_t)a.path);
-        } else {
-            launch_features(d, p.keys, qoff, nq, ix->R, (float *)c->feat.p, (uint64_t *)c->M.p, s);
-            const int slot = nq > 1 ? 1 : 0;
-            if ((a.path == AB_PATH_LEARNED || a.path == AB_PATH_AUTO) && c->mlp[slot].loaded) {
-                AB_TRY(mlp_decide(c, slot, (const float *)c->feat.p, nq, sel));
-            } else {
-                const double tau = nq > 1 ? c->tau_batch : c->tau;
-                launch_decide_threshold((const uint64_t *)c->M.p, nq, tau * (double)ix->R, sel, s);
-            }
-        }
-        launch_assign_groups(sel, nq, p.G, (uint32_t *)c->qgroup.p, (uint32_t *)c->grp_q.p, (uint32_t *)c->grp_n.p,
-                             (uint32_t *)c->plan.p, s);
-    }
-    {
-        TimedSpan ts(c, AB_K_DENSE_MASK);
-        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
-                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 0, s);
-    }
     const uint64_t *Kd = p.graph ? qoff + nq : nullptr;  // qoff[nq] = the key count
     SparseArgs sa{};
     sa.keys = p.keys;
@@ -123,30 +101,54 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     sa.cap = p.sp_cap;
     sa.out = p.bm;
     sa.out_stride = p.stride;
+    {
+        TimedSpan ts(c, AB_K_DECIDE);
+        // the per-query record counts M (features) split the sparse queries
+        const uint64_t *M = nullptr;
+        if (a.path == AB_PATH_DENSE || a.path == AB_PATH_SPARSE) {
+            hipLaunchKernelGGL(k_fill_u8, dim3((nq + 255) / 256), dim3(256), 0, s, sel, nq, (uint8_t)a.path);
+            if (a.path == AB_PATH_SPARSE) {
+                launch_features(d, p.keys, qoff, nq, ix->R, (float *)c->feat.p, (uint64_t *)c->M.p, s);
+                M = (const uint64_t *)c->M.p;
+            }
+        } else {
+            launch_features(d, p.keys, qoff, nq, ix->R, (float *)c->feat.p, (uint64_t *)c->M.p, s);
+            M = (const uint64_t *)c->M.p;
+            const int slot = nq > 1 ? 1 : 0;
+            if ((a.path == AB_PATH_LEARNED || a.path == AB_PATH_AUTO) && c->mlp[slot].loaded) {
+                AB_TRY(mlp_decide(c, slot, (const float *)c->feat.p, nq, sel));
+            } else {
+                const double tau = nq > 1 ? c->tau_batch : c->tau;
+                launch_decide_threshold((const uint64_t *)c->M.p, nq, tau * (double)ix->R, sel, s);
+            }
+        }
+        launch_assign_groups(sel, M, nq, p.G, (uint32_t *)c->qgroup.p, (uint32_t *)c->grp_q.p, (uint32_t *)c->grp_n.p,
+                             (uint32_t *)c->plan.p, sa, s);
+    }
+    if (!p.sp_cap) {
+        // the atomic path ORs into zeroed bitmaps; slice queries write theirs whole
+        TimedSpan ts(c, AB_K_SPARSE_ZERO);
+        launch_zero_sparse(p.bm, p.stride, p.words, sa.flag, nq, kSpAtomic, s);
+    }
+    {
+        TimedSpan ts(c, AB_K_DENSE_MASK);
+        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
+                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 0, s);
+    }
     if (p.Kcap) {
         {
             TimedSpan ts(c, AB_K_SPARSE_PLAN);
             launch_sparse_plan(d, p.keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
             launch_scan_u64((const uint64_t *)c->cnt.p, (uint64_t *)c->koff.p, p.Kcap, (uint64_t *)c->parts.p, s,
                             Kd);
-            launch_sparse_split(sel, sa, s);
-        }
-        {
-            // only the atomic path needs zeroed bitmaps; slice queries write theirs whole
-            TimedSpan ts(c, AB_K_SPARSE_ZERO);
-            launch_zero_sparse(p.bm, p.stride, p.words, sa.flag, nq, kSpAtomic, s);
         }
         TimedSpan ts(c, AB_K_SPARSE_UNION);
         HIP_TRY(launch_sparse_route(d, sa, c->sp_grid, s));
-    } else {
-        // no keys at all: every sparse query's result is empty
-        TimedSpan ts(c, AB_K_SPARSE_ZERO);
-        launch_zero_sparse(p.bm, p.stride, p.words, sel, nq, AB_PATH_SPARSE, s);
     }
     {
         TimedSpan ts(c, AB_K_FUSED_UNION);
         FusedArgs f{};
-        if (p.Kcap) {
+        {
             f.sp_tot = sa.tot;
             f.sp_slist = sa.slist;
             f.sp_cpre = sa.cpre;
@@ -253,7 +255,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     AB_TRY(c->parts.ensure(scan_parts_needed(p.Kcap) * 8 + 16));
     if (p.graph) AB_TRY(c->gkeys.ensure(p.Kcap * 4 + 16));
     AB_TRY(c->ccnt.ensure((size_t)nq * nchunk * 4 + 4));
-    if (p.Kcap) {
+    {
         AB_TRY(c->sp_flag.ensure(nq + 16));
         AB_TRY(c->sp_cpre.ensure(((size_t)nq + 1) * 4));
         AB_TRY(c->sp_scpre.ensure((size_t)nq * 4 + 4));

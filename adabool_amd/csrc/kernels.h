@@ -63,9 +63,8 @@ uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t thr
 hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, const void *mask,
                               const uint32_t *anybm, const DenseGroup &g, uint32_t *out,
                               uint64_t out_stride, uint64_t out_words, bool use_col, hipStream_t s);
-// Device-dispatched batch: group assignment, sparse zero-fill, group masks.
-void launch_assign_groups(const uint8_t *sel, uint32_t nq, uint32_t G, uint32_t *qgroup, uint32_t *grp_q,
-                          uint32_t *grp_n, uint32_t *plan, hipStream_t s);
+
+// Device-dispatched batch: sparse zero-fill, group masks.
 // Zeroes the bitmaps of the queries with sel[q] == match.
 void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq,
                         uint8_t match, hipStream_t s);
@@ -76,9 +75,9 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
 // A sparse query's bitmap is cut into value slices of 2^kSpSliceShift values
 // (32 KB of bits); a route pass sorts the query's records by slice, and one
 // work item per (query, slice) ORs that slice's records into LDS and writes
-// the slice once.  Queries beyond the context's record budget, and indices
-// with more slices or records than the references encode, take the atomic
-// path instead (zero-fill, then one global atomic per touched word).
+// the slice once.  Queries beyond the context's record budget run dense
+// instead; indices with more slices or records than the references encode
+// take the atomic path (zero-fill, then one global atomic per touched word).
 constexpr uint32_t kSpSliceShift = 18;
 constexpr uint32_t kSpSliceWords = 1u << (kSpSliceShift - 5);
 constexpr uint32_t kSpMaxSlices = 1024;
@@ -106,8 +105,11 @@ struct SparseArgs {
     uint32_t *out;
     uint64_t out_stride;
 };
-// One work-group: flags, prefixes and totals of the sparse queries.
-void launch_sparse_split(const uint8_t *sel, const SparseArgs &a, hipStream_t s);
+// Device-dispatched batch, one work-group: the sparse queries' path (slice,
+// atomic, or dense past the record budget; M from the features) with their
+// route-chunk prefixes and refs bases, then the dense groups.
+void launch_assign_groups(uint8_t *sel, const uint64_t *M, uint32_t nq, uint32_t G, uint32_t *qgroup,
+                          uint32_t *grp_q, uint32_t *grp_n, uint32_t *plan, const SparseArgs &a, hipStream_t s);
 // Route chunks: slice queries' records sorted into refs; atomic queries'
 // records OR-ed into their (zeroed) bitmaps.  Persistent over `grid`.
 hipError_t launch_sparse_route(const DevIndex &ix, const SparseArgs &a, uint32_t grid, hipStream_t s);

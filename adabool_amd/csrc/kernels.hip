@@ -857,52 +857,76 @@ hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, co
 // device-dispatched batches: group assignment, per-group query masks, and one
 // persistent union launch over dense (group, tile) items and sparse chunks
 // ---------------------------------------------------------------------------
-// Dense queries in query order get rank r -> group r / G, slot r % G; qgroup[q]
-// = group * kGroupSlots + slot (kNoGroup for sparse queries); plan = {groups,
-// dense}.
-__global__ __launch_bounds__(1024) void k_assign_groups(const uint8_t *__restrict__ sel, uint32_t nq, uint32_t G,
-                                                        uint32_t *qgroup, uint32_t *grp_q, uint32_t *grp_n,
-                                                        uint32_t *plan)
+// One work-group of 1024 threads, queries in order.
+// Sparse queries first: with a record budget (a.cap > 0), a sparse query takes
+// the write-once slice path while the running total of sparse records M (from
+// the features) fits the budget -- its refs start at that running total --
+// and runs dense instead past it (sel[q] rewritten); without a budget it takes
+// the atomic path.  Slice and atomic queries get ceil(M / chunk) route chunks.
+// Then dense queries in query order get rank r -> group r / G, slot r % G;
+// qgroup[q] = group * kGroupSlots + slot (kNoGroup otherwise); plan = {groups,
+// dense}; a.tot = {route chunks, slice queries, slice chunks}.
+__global__ __launch_bounds__(1024) void k_assign_groups(uint8_t *__restrict__ sel, const uint64_t *__restrict__ M,
+                                                        uint32_t nq, uint32_t G, uint32_t *qgroup, uint32_t *grp_q,
+                                                        uint32_t *grp_n, uint32_t *plan, SparseArgs a)
 {
-    __shared__ uint32_t sh[16];
-    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    uint32_t running = 0;
+    __shared__ unsigned long long sh[16];
+    unsigned long long run_d = 0, run_m = 0, run_ch = 0, run_sq = 0, run_sch = 0, tot;
     for (uint32_t base = 0; base < nq; base += 1024) {
         const uint32_t q = base + threadIdx.x;
-        const uint32_t d = (q < nq && sel[q] == AB_PATH_DENSE) ? 1u : 0u;
-        const uint32_t incl = wave_incl_scan(d);
-        if (lane == 63) sh[wave] = incl;
-        __syncthreads();
-        uint32_t before = 0, all = 0;
-        for (uint32_t w = 0; w < 16; ++w) {
-            if (w < wave) before += sh[w];
-            all += sh[w];
-        }
-        __syncthreads();
-        const uint32_t rank = running + before + incl - d;
-        if (q < nq) {
+        const bool in = q < nq;
+        const uint8_t s0 = in ? sel[q] : (uint8_t)0;
+        const bool sparse = in && s0 == AB_PATH_SPARSE;
+        const unsigned long long m = (sparse && M) ? M[q] : 0;
+        const unsigned long long m_before = run_m + block_excl_scan(m, sh, &tot);
+        run_m += tot;
+        const bool slice = sparse && a.cap && m_before + m <= a.cap;
+        const bool atomic = sparse && !a.cap;
+        const bool d = (in && s0 == AB_PATH_DENSE) || (sparse && !slice && !atomic);
+        if (in && d && s0 != AB_PATH_DENSE) sel[q] = AB_PATH_DENSE;
+        const unsigned long long nch = (slice || atomic) ? (m + a.chunk - 1) / a.chunk : 0;
+        const unsigned long long ch_before = run_ch + block_excl_scan(nch, sh, &tot);
+        run_ch += tot;
+        const unsigned long long rank_s = run_sq + block_excl_scan(slice ? 1ull : 0ull, sh, &tot);
+        run_sq += tot;
+        const unsigned long long sch_before = run_sch + block_excl_scan(slice ? nch : 0ull, sh, &tot);
+        run_sch += tot;
+        const unsigned long long rank = run_d + block_excl_scan(d ? 1ull : 0ull, sh, &tot);
+        run_d += tot;
+        if (in) {
             if (d) {
-                const uint32_t g = rank / G, b = rank - g * G;
+                const uint32_t g = (uint32_t)(rank / G), b = (uint32_t)(rank - (unsigned long long)g * G);
                 qgroup[q] = g * kGroupSlots + b;
                 grp_q[g * kGroupSlots + b] = q;
             } else {
                 qgroup[q] = kNoGroup;
             }
+            a.flag[q] = slice ? kSpSlice : (atomic ? kSpAtomic : kSpNone);
+            a.cpre[q] = (uint32_t)ch_before;
+            if (slice) {
+                a.scpre[q] = (uint32_t)sch_before;
+                a.rbase[q] = m_before;
+                a.slist[rank_s] = q;
+            }
         }
-        running += all;
     }
-    const uint32_t ng = (running + G - 1) / G;
-    for (uint32_t g = threadIdx.x; g < ng; g += 1024) grp_n[g] = (running - g * G) < G ? running - g * G : G;
+    const uint32_t dense = (uint32_t)run_d;
+    const uint32_t ng = (dense + G - 1) / G;
+    for (uint32_t g = threadIdx.x; g < ng; g += 1024) grp_n[g] = (dense - g * G) < G ? dense - g * G : G;
     if (threadIdx.x == 0) {
         plan[0] = ng;
-        plan[1] = running;
+        plan[1] = dense;
+        a.cpre[nq] = (uint32_t)run_ch;
+        a.tot[0] = (uint32_t)run_ch;
+        a.tot[1] = (uint32_t)run_sq;
+        a.tot[2] = (uint32_t)run_sch;
     }
 }
 
-void launch_assign_groups(const uint8_t *sel, uint32_t nq, uint32_t G, uint32_t *qgroup, uint32_t *grp_q,
-                          uint32_t *grp_n, uint32_t *plan, hipStream_t s)
+void launch_assign_groups(uint8_t *sel, const uint64_t *M, uint32_t nq, uint32_t G, uint32_t *qgroup,
+                          uint32_t *grp_q, uint32_t *grp_n, uint32_t *plan, const SparseArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_assign_groups, dim3(1), dim3(1024), 0, s, sel, nq, G, qgroup, grp_q, grp_n, plan);
+    hipLaunchKernelGGL(k_assign_groups, dim3(1), dim3(1024), 0, s, sel, M, nq, G, qgroup, grp_q, grp_n, plan, a);
 }
 
 __global__ __launch_bounds__(256) void k_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words,
@@ -985,62 +1009,9 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
 }
 
 // ---------------------------------------------------------------------------
-// write-once sparse: split, route, and the per-(query, slice) emit
+// write-once sparse: route and the per-(query, slice) emit (the split into
+// slice, atomic and overflow-to-dense queries is made by k_assign_groups)
 // ---------------------------------------------------------------------------
-// One work-group of 1024 threads, queries in order.  A sparse query gets
-// ceil(M / chunk) route chunks (M = its records, from the key prefix koff);
-// while the running record total of sparse queries stays within cap it takes
-// the slice path (its refs start at that running total), and every later one
-// the atomic path.  Totals: tot[0] route chunks, tot[1] slice queries,
-// tot[2] slice chunks.
-__global__ __launch_bounds__(1024) void k_sparse_split(const uint8_t *__restrict__ sel, SparseArgs a)
-{
-    __shared__ unsigned long long sh[16];
-    const uint64_t K = a.Kd ? *a.Kd : a.K;
-    unsigned long long run_ch = 0, run_m = 0, run_sq = 0, run_sch = 0;
-    for (uint32_t base = 0; base < a.nq; base += 1024) {
-        const uint32_t q = base + threadIdx.x;
-        const bool in = q < a.nq;
-        const bool sparse = in && sel[q] == AB_PATH_SPARSE;
-        uint64_t m = 0;
-        if (sparse && K) {
-            const uint64_t j0 = a.qoff[q], j1 = a.qoff[q + 1];
-            m = a.koff[j1] - a.koff[j0];
-        }
-        const unsigned long long nch = sparse ? (m + a.chunk - 1) / a.chunk : 0;
-        unsigned long long tot;
-        const unsigned long long ch_before = run_ch + block_excl_scan(nch, sh, &tot);
-        run_ch += tot;
-        const unsigned long long m_before = run_m + block_excl_scan(sparse ? m : 0, sh, &tot);
-        run_m += tot;
-        const bool slice = sparse && a.cap && m_before + m <= a.cap;
-        const unsigned long long rank = run_sq + block_excl_scan(slice ? 1ull : 0ull, sh, &tot);
-        run_sq += tot;
-        const unsigned long long sch_before = run_sch + block_excl_scan(slice ? nch : 0ull, sh, &tot);
-        run_sch += tot;
-        if (in) {
-            a.flag[q] = sparse ? (slice ? kSpSlice : kSpAtomic) : kSpNone;
-            a.cpre[q] = (uint32_t)ch_before;
-            if (slice) {
-                a.scpre[q] = (uint32_t)sch_before;
-                a.rbase[q] = m_before;
-                a.slist[rank] = q;
-            }
-        }
-    }
-    if (threadIdx.x == 0) {
-        a.cpre[a.nq] = (uint32_t)run_ch;
-        a.tot[0] = (uint32_t)run_ch;
-        a.tot[1] = (uint32_t)run_sq;
-        a.tot[2] = (uint32_t)run_sch;
-    }
-}
-
-void launch_sparse_split(const uint8_t *sel, const SparseArgs &a, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_sparse_split, dim3(1), dim3(1024), 0, s, sel, a);
-}
-
 uint32_t sparse_route_lds_bytes(uint32_t chunk)
 {
     // key-offset window | slice histogram | staged refs | staged slices
@@ -1132,7 +1103,11 @@ hipError_t launch_sparse_route(const DevIndex &ix, const SparseArgs &a, uint32_t
 __device__ __forceinline__ void emit_ref_lds(const DevIndex &ix, uint32_t ref, uint32_t *slice, uint32_t w0)
 {
     const uint32_t idx = ref & ((1u << kSpTypeShift) - 1);
-    auto put = [&](uint32_t word, uint32_t bits) { atomicOr(&slice[word - w0], bits); };
+    // a routed record lies in its slice; the bound keeps LDS accesses inside it regardless
+    auto put = [&](uint32_t word, uint32_t bits) {
+        const uint32_t o = word - w0;
+        if (o < kSpSliceWords) atomicOr(&slice[o], bits);
+    };
     uint32_t w[8];
     switch (ref >> kSpTypeShift) {
     case AB_ARRAY1: {

@@ -76,24 +76,60 @@ def measured_traffic(path: str, kclass: str):
 
 
 def cpu_model() -> str:
+    """lscpu's model name, with the CPU family and model numbers (a
+    virtualized host may report only the vendor's product line)."""
+    name, fam, mod = "unknown", None, None
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and name == "unknown":
+                name = v
+            elif k == "cpu family" and fam is None:
+                fam = v
+            elif k == "model" and mod is None:
+                mod = v
     except OSError:
         pass
-    return "unknown"
+    return f"{name} (family {fam}, model {mod})" if fam is not None else name
 
 
-def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float):
+def host_threads() -> int:
+    """Host cores this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float, sample=None, more=()):
     """Oracle (plain C, CSR-by-key union -> bitmap + ids) on the host cores,
     over a bounded sample of the same queries, about budget_s of wall time
     (a prefix, or whole passes over all of them; one thread per core across
-    queries, and one thread alone); also checks the GPU counts against it."""
+    queries, and one thread alone).  As the checker it also compares: the
+    GPU counts of those queries; the bitmaps and ids of `sample` (host copies
+    of the first queries' GPU results: bitmaps [n, stride], ids, counts) by
+    per-query digests; and, for each (name, keys, qoff, counts) in `more`,
+    the GPU counts of every query."""
     import oracle
 
-    nthreads = min(16, os.cpu_count() or 1)
+    nthreads = host_threads()
     csr = oracle.CSR(pairs_k, pairs_v, spec.nkeys, spec.nvalues)
+    checks = {}
+    if sample is not None:
+        bms, ids, cnts = sample
+        n = bms.shape[0]
+        wc, wb, wi = csr.union_batch_digest(keys[: int(qoff[n])], qoff[: n + 1], nthreads)
+        off = np.concatenate([[0], np.cumsum(cnts.astype(np.uint64))[:-1]]).astype(np.uint64)
+        gi, asc = oracle.ids_digest(ids, off, cnts, nthreads)
+        checks["sample_queries"] = n
+        checks["sample_bitmaps_match"] = bool(np.array_equal(oracle.bitmap_digest(bms, 0, nthreads), wb))
+        checks["sample_ids_match"] = bool(np.array_equal(cnts.astype(np.uint64), wc) and asc.all()
+                                          and np.array_equal(gi, wi))
+    for name, mk, mo, mc in more:
+        _, oc, _, _ = csr.union_batch(mk, mo, nthreads)
+        checks[f"{name}_counts_match"] = bool(np.array_equal(oc.astype(np.int64), np.asarray(mc, np.int64)))
+        checks[f"{name}_queries"] = int(mo.size - 1)
     nq = qoff.size - 1
     # calibrate on a few queries, then size the sample to ~budget_s of work
     n0 = min(nq, 2 * nthreads)
@@ -118,7 +154,8 @@ def cpu_baseline(pairs_k, pairs_v, spec, keys, qoff, counts_gpu, budget_s: float
     return {"value": ns * passes / t, "unit": "queries/sec", "cores": nthreads, "kind": "port",
             "sample": f"{what} C2 queries, oracle/oracle.c or_union_batch, {nthreads} threads, "
                       f"{t:.1f} s", "counts_match_gpu": ok, "cpu_model": cpu_model(),
-            "host_cpus_visible": os.cpu_count(),
+            "host_cpus_visible": os.cpu_count(), "compiled": "gcc -O3 -march=x86-64-v4 (AVX-512)",
+            "checks": checks,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "single_thread": {"value": n1 / t1, "queries": n1, "p50_ms": float(np.percentile(lat1, 50)),
                               "p99_ms": float(np.percentile(lat1, 99))},
@@ -148,6 +185,117 @@ def coo_scan_c1(reps: int = 20):
             "queries": reps, "p50_ms": float(np.median(lat)) * 1e3, "matches_csr_union": ok}
 
 
+def c3_leg(ab, torch, ctx, ix, spec, pk, path, steps: int = 5, nq: int = 4096):
+    """C3 mix on the C2 filter: one batch of 4096 power-law queries (1..50000
+    Zipf-weighted keys) per step through the same autotuned context (graph
+    mode), ids packed at offsets from each query's id bound.  Steps run one
+    at a time; the union's roofline counts each query's list bytes and its
+    bitmap (no sharing credit) over the sparse route + fused union time."""
+    from adabool_amd import configs
+
+    qk, qo = configs.make_queries(spec, configs.C3_Q, nq)
+    nnz_key = np.bincount(pk, minlength=spec.nkeys).astype(np.int64)
+    valid = qk < spec.nkeys
+    per_key = np.where(valid, nnz_key[np.where(valid, qk, 0)], 0)
+    cs = np.concatenate([[0], np.cumsum(per_key)])
+    ub = np.minimum(cs[qo[1:].astype(np.int64)] - cs[qo[:-1].astype(np.int64)], spec.nvalues)
+    room = (ub + 3) // 4 * 4
+    off = np.concatenate([[0], np.cumsum(room)[:-1]]).astype(np.uint64)
+    d_keys = torch.from_numpy(qk.view(np.int32)).cuda()
+    d_ids = torch.empty(int(room.sum()) + 4, dtype=torch.int32, device="cuda")
+    d_cnt = torch.empty(nq, dtype=torch.int32, device="cuda")
+    paths = None
+    for _ in range(2):  # first call runs as launches, the second is captured
+        paths = ctx.query_batch_packed(d_keys, qo, path, d_ids, off, None, d_cnt)
+    ctx.sync()
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        ctx.query_batch_packed(d_keys, qo, path, d_ids, off, None, d_cnt)
+        ctx.sync()
+        lat.append((time.perf_counter() - s0) * 1e3)
+    wall = time.perf_counter() - t0
+    counts = d_cnt.cpu().numpy().view(np.uint32).astype(np.int64)
+    ctx.timing(reset=True)
+    ctx.set_timing(True)
+    for _ in range(2):
+        ctx.query_batch_packed(d_keys, qo, path, d_ids, off, None, d_cnt)
+    ctx.sync()
+    kt = ctx.timing(reset=True)
+    ctx.set_timing(False)
+    union_ms = (kt["fused_union"][0] + kt["sparse_union"][0]) / 2
+    alg = float((list_bytes_per_query(ix, qk, qo) + ix.stride_bytes).sum())
+    out = {"workload": f"C2 filter, {nq} C3-mix queries (power law 1.5 on [1, 50000], Zipf-weighted keys) per "
+                       f"batch; ids + counts", "queries_per_step": nq, "steps": steps,
+           "qps": round(nq * steps / wall, 1), "batch_p50_ms": float(np.percentile(lat, 50)),
+           "batch_p99_ms": float(np.percentile(lat, 99)), "dense_queries": int((paths == ab.AB_PATH_DENSE).sum()),
+           "ids_per_query_mean": float(counts.mean()),
+           "kernel_ms_per_step": {k: round(v[0] / 2, 4) for k, v in kt.items() if v[1]},
+           "union_roofline": {"bound": "hbm", "achieved": round(alg / (union_ms * 1e-3) / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(alg / (union_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                              "bytes_per_step": alg, "union_ms": round(union_ms, 4)}}
+    log(f"C3 mix batch {nq}: {out['qps']:.0f} q/s, batch p50 {out['batch_p50_ms']:.2f} ms, "
+        f"dense {out['dense_queries']}, union {union_ms:.2f} ms")
+    del d_ids, d_keys, d_cnt
+    torch.cuda.empty_cache()
+    return out, (qk, qo, counts)
+
+
+def c5_leg(ab, torch, model_path: str, nq: int = 8192, rates=(0.0, 10000.0, 50000.0)):
+    """C5 serving: the 16M x 4M filter (~400M pairs) built on the device and
+    resident, C3-mix requests replayed through the serving loop (1024-query
+    batches, two in flight, memory-based admission into a 64 GiB result
+    arena, learned dispatch): saturation (all at once) and Poisson arrivals
+    at 10k and 50k queries/s, with p50/p99 latency from each arrival."""
+    from adabool_amd import configs
+
+    spec = configs.C5
+    t = time.time()
+    pk, pv = configs.make_filter(spec)
+    dk = torch.from_numpy(pk.view(np.int32)).cuda()
+    dv = torch.from_numpy(pv.view(np.int32)).cuda()
+    del pk, pv
+    ix = ab.Index.build_device(dk, dv, spec.nkeys, spec.nvalues, 10)
+    torch.cuda.synchronize()
+    del dk, dv
+    torch.cuda.empty_cache()
+    info = ix.info
+    qk, qo = configs.make_queries(spec, configs.C3_Q, nq)
+    log(f"C5 index: {info['nnz']} pairs, device {info['device_bytes'] / 1e9:.2f} GB ({time.time() - t:.1f}s)")
+    srv = ab.Server(ix, max_batch=1024, max_wait_us=200, inflight=2, arena_bytes=64 << 30, batch_mode=1)
+    runs = []
+    try:
+        if os.path.exists(model_path):
+            srv.load_model(model_path)
+        for c in srv.contexts:  # the serving autotune's choice for this filter
+            c.set_dense_group(128)
+            c.set_dense_threads(512)
+            c.set_sparse_chunk(1024)
+        srv.replay(qk[: int(qo[2048])], qo[:2049], 0.0, 7)
+        srv.stats(reset=True)
+        ref = None
+        for rate in rates:
+            counts, lat, wall = srv.replay(qk, qo, rate, 11)
+            st = srv.stats(reset=True)
+            ref = counts if ref is None else ref
+            runs.append({"offered_qps": rate or None, "achieved_qps": round(nq / wall, 1),
+                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                         "mean_batch": round(st["mean_batch"], 1), "dense_queries": int(st["dense_queries"]),
+                         "counts_equal_saturation_run": bool(np.array_equal(counts, ref))})
+            log(f"C5 serving, offered {rate or 'all at once'}: {runs[-1]['achieved_qps']:.0f} q/s, "
+                f"p50 {runs[-1]['p50_ms']:.2f} ms, p99 {runs[-1]['p99_ms']:.2f} ms")
+    finally:
+        srv.close()
+        ix.close()
+        torch.cuda.empty_cache()
+    return {"workload": f"C5: {spec.nvalues} values x {spec.nkeys} keys, {info['nnz']} pairs (device-built, "
+                        f"value tile 2^10); {nq} C3-mix requests", "saturation_qps": runs[0]["achieved_qps"],
+            "ids_per_query_mean": float(ref.mean()), "runs": runs}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +322,8 @@ def main():
     ap.add_argument("--tile-shift", type=int, default=0, help="value tile of the built index (0: automatic)")
     ap.add_argument("--tile-shifts", default="9,10,12",
                     help="value-tile sizes (log2) the autotuner also tries, one device-built index each")
+    ap.add_argument("--no-c3", action="store_true", help="skip the C3-mix batch leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the C5 serving leg")
     ap.add_argument("--streams", type=int, default=2,
                     help="contexts (one stream, graph and output buffers each) that consecutive timed steps "
                          "alternate over, so one step's compaction runs beside the next one's union; 1: every "
@@ -467,10 +617,22 @@ def main():
         log(f"per-query learned: {learned_pq['qps']:.0f} q/s, p50 {learned_pq['p50_ms']:.3f} ms "
             f"p99 {learned_pq['p99_ms']:.3f} ms")
 
+    c3 = c3_data = None
+    if not args.no_c3 and args.scale == 1.0:
+        c3, c3_data = c3_leg(ab, torch, ctx, ix, spec, pk, path)
+
     cpu = None
     if not args.no_cpu:
+        # host copies of the first queries' GPU results of the timed run, for the checker
+        ns = min(64, nq)
+        smp_bm = d_bm[: ns * sw].cpu().numpy().view(np.uint32).reshape(ns, sw)
+        smp_ids = np.concatenate([d_ids[i * nv: i * nv + int(counts[i])].cpu().numpy().view(np.uint32)
+                                  for i in range(ns)])
+        more = [("c3_mix",) + c3_data] if c3_data is not None else []
         t = time.time()
-        cpu = cpu_baseline(pk, pv, spec, qk, qo, counts, args.cpu_budget)
+        cpu = cpu_baseline(pk, pv, spec, qk, qo, counts, args.cpu_budget,
+                           sample=(smp_bm, smp_ids, counts[:ns].astype(np.uint32)), more=more)
+        del smp_bm, smp_ids
         log(f"cpu baseline: {cpu['value']:.1f} q/s on {cpu['cores']} threads ({time.time() - t:.1f}s), "
             f"counts match: {cpu['counts_match_gpu']}")
 
@@ -512,7 +674,10 @@ def main():
         "kernel_ms_per_step": {k: round(v[0] / ksteps, 4) for k, v in kt.items() if v[1]},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "c3_mix_batch4096": c3,
     }
+    if not args.no_c5 and args.scale == 1.0:
+        out["c5_serving"] = c5_leg(ab, torch, args.model)
     print(json.dumps(out), flush=True)
 
 
