@@ -717,10 +717,13 @@ __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, co
     const uint32_t tw = tv >> 5;
     const uint32_t tile_word0 = tile * tw;
     const uint32_t ntw = B * tw;
-    M *col = (MODE != kModeBitmap && use_col) ? reinterpret_cast<M *>(tiles + ntw) : nullptr;
-    for (uint32_t i = threadIdx.x; i < ntw; i += NT) tiles[i] = 0;
+    // with column masks the rows are never accumulated: the columns are
+    // transposed straight into the output, so no row tiles are kept in LDS
+    M *col = (MODE != kModeBitmap && use_col) ? reinterpret_cast<M *>(tiles) : nullptr;
     if (col)
         for (uint32_t i = threadIdx.x; i < tv; i += NT) col[i] = M{};
+    else
+        for (uint32_t i = threadIdx.x; i < ntw; i += NT) tiles[i] = 0;
     __syncthreads();
 
     const uint32_t lane = lane_id();
@@ -768,29 +771,40 @@ __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, co
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     __syncthreads();
+    const uint64_t nwq = out_words > tile_word0 ? ((out_words - tile_word0) < tw ? out_words - tile_word0 : tw) : 0;
     if (col) {
-        // columns -> rows: lane l holds the query set of value 64*gi + l; a
-        // 64 x 64 bit transpose across the wave gives lane b query b's 64 bits
-        // of those values (64 queries at a time).  Six butterfly stages
-        // replace 64 ballots: the union went from 2.78 to 2.47 ms at C2.
+        // columns -> output rows: lane l holds the query set of value 64*gi + l;
+        // a 64 x 64 bit transpose across the wave gives lane b query b's 64
+        // bits of those values (64 queries at a time), two words of its row
+        // that the lane stores at once.  Every word of the tile is written,
+        // for every query of the group (zeros included).
+        const bool pair = ((out_stride & 1) == 0) && ((reinterpret_cast<uintptr_t>(out) & 7) == 0);
         for (uint32_t gi = wave; gi < (tv >> 6); gi += NW) {
             const M cv = col[gi * 64 + lane];
+            const uint32_t w = 2 * gi;  // first of the two words, within the tile's row
             for (uint32_t c = 0; c * 64 < B; ++c) {
                 const unsigned long long qv = mask_word(cv, c);
-                if (__ballot(qv != 0) == 0) continue;
                 const uint32_t nb = B - c * 64 < 64 ? B - c * 64 : 64;
-                const unsigned long long mine = transpose64(qv, lane);
-                if (lane < nb) {
-                    uint32_t *row = tiles + (c * 64 + lane) * tw;
-                    row[2 * gi] |= (uint32_t)mine;
-                    row[2 * gi + 1] |= (uint32_t)(mine >> 32);
+                unsigned long long mine = 0;
+                if (__ballot(qv != 0) != 0) mine = transpose64(qv, lane);
+                if (lane < nb && w < nwq) {
+                    uint32_t *dst = out + (uint64_t)qids[c * 64 + lane] * out_stride + tile_word0 + w;
+                    if (w + 1 < nwq) {
+                        if (pair) {
+                            *reinterpret_cast<uint2 *>(dst) = make_uint2((uint32_t)mine, (uint32_t)(mine >> 32));
+                        } else {
+                            dst[0] = (uint32_t)mine;
+                            dst[1] = (uint32_t)(mine >> 32);
+                        }
+                    } else {
+                        dst[0] = (uint32_t)mine;
+                    }
                 }
             }
         }
-        __syncthreads();
+        return;
     }
     // write-once: every word of the tile, for every query of the group
-    const uint64_t nwq = out_words > tile_word0 ? ((out_words - tile_word0) < tw ? out_words - tile_word0 : tw) : 0;
     if ((out_stride & 3) == 0 && (nwq & 3) == 0) {
         const uint32_t n4 = (uint32_t)(nwq >> 2), tw4 = tw >> 2;
         const uint4 *src = reinterpret_cast<const uint4 *>(tiles);
@@ -818,10 +832,13 @@ __global__ __launch_bounds__(NT) void k_dense_tiles(DevIndex ix, const void *__r
 
 uint32_t dense_lds_bytes(int mode, uint32_t B, uint32_t tile_shift, uint32_t threads, bool use_col)
 {
-    const uint32_t col = (mode == kModeBitmap || !use_col)
-                             ? 0u
-                             : (1u << tile_shift) * (mode == kModeMask32 ? 4u : (mode == kModeMask64 ? 8u : (mode == kModeMask128 ? 16u : 32u)));
-    return (threads / 64) * kBlockRecs * 4u + B * (1u << (tile_shift - 5)) * 4u + col;
+    // key staging, then either the column masks (transposed straight to the
+    // output) or the row tiles
+    const uint32_t staging = (threads / 64) * kBlockRecs * 4u;
+    if (mode != kModeBitmap && use_col)
+        return staging + (1u << tile_shift) *
+                             (mode == kModeMask32 ? 4u : (mode == kModeMask64 ? 8u : (mode == kModeMask128 ? 16u : 32u)));
+    return staging + B * (1u << (tile_shift - 5)) * 4u;
 }
 
 template <int MODE>
