@@ -8,15 +8,18 @@ each is timed.  Two label sets, one per model slot:
   (enqueue -> stream synchronize, host wall clock), best of `repeats`;
 * batch (MODEL_BATCH): the query's share of a batch, from HIP-event device
   times of device-dispatched batches (one fused union launch) over random
-  groups of 128 queries, each group run once on either path: dense -- the
+  groups of queries, each group run once on either path: dense -- the
   group's mask and union kernels divided by the group size (one pass over the
-  index serves the group); sparse -- the plan divided by the group size, plus
+  index serves the group; groups of 256, the width the batches are tuned
+  to); sparse -- the plan divided by the group size, plus
   the route pass and the slice items in proportion to the query's records
   M(q) (the work of both follows the records).  Timing a sparse query alone would charge it
   a whole launch that a batch shares.
 
 Each model (12 features -> 64 -> 64 -> 2, log milliseconds of dense and
 sparse) is trained by the library's own HIP kernels (ab_mlp_train); the
+batch network is early-stopped on the regret of a seeded tenth of the
+training queries held back for validation.  The
 threshold rule (dense iff M(q) >= tau * R, tau fit on the same training
 timings) is both the fallback and the yardstick: regret = sum of the chosen
 path's time over the sum of the per-query best time, minus one.
@@ -101,7 +104,7 @@ def time_both(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, repea
 
 
 def time_batch_shares(ctx: Context, ix: Index, keys: np.ndarray, qoff: np.ndarray, M: np.ndarray, seed: int,
-                      repeats: int = 2, group: int = 128):
+                      repeats: int = 2, group: int = 256):
     """Per query: device ms of its share of a batch on each path (see module doc)."""
     import torch
 
@@ -167,12 +170,30 @@ def regret(choice_dense: np.ndarray, t: np.ndarray) -> float:
     return float(chosen.sum() / np.minimum(t[:, 0], t[:, 1]).sum() - 1.0)
 
 
-def _train(ctx: Context, slot: int, F, T, tr, epochs: int):
+def _train(ctx: Context, slot: int, F, T, tr, epochs: int, va=None, rounds: int = 1):
+    """Trains the slot's network on the rows tr.  With a validation mask va,
+    training runs in `rounds` rounds of epochs // rounds epochs and keeps the
+    parameters of the round with the lowest validation regret (early
+    stopping).  Returns the choices on every row, the loss curve and the
+    validation regret of the kept parameters."""
     ctx.clear_model(slot)
     Y = np.log(T).astype(np.float32)
-    curve = ctx.train_model(F[tr], Y[tr], epochs=epochs, batch=256, lr=2e-3, seed=7 + slot, slot=slot)
+    if va is None or rounds <= 1:
+        curve = ctx.train_model(F[tr], Y[tr], epochs=epochs, batch=256, lr=2e-3, seed=7 + slot, slot=slot)
+        _, sel = ctx.predict(F, slot=slot)
+        return sel == AB_PATH_DENSE, curve, None
+    per = max(1, epochs // rounds)
+    best, best_r, curve = None, np.inf, []
+    for r in range(rounds):
+        curve.extend(ctx.train_model(F[tr], Y[tr], epochs=per, batch=256, lr=2e-3, seed=7 + slot + 1000 * r,
+                                     slot=slot).tolist())
+        _, sel = ctx.predict(F[va], slot=slot)
+        rv = regret(sel == AB_PATH_DENSE, T[va])
+        if rv < best_r:
+            best_r, best = rv, ctx.get_model(slot)
+    ctx.set_model(*best, slot=slot)
     _, sel = ctx.predict(F, slot=slot)
-    return sel == AB_PATH_DENSE, curve
+    return sel == AB_PATH_DENSE, np.asarray(curve, np.float32), best_r
 
 
 def main(argv: List[str] = None):
@@ -181,6 +202,7 @@ def main(argv: List[str] = None):
     ap.add_argument("--data-out", default="cost_timings_c4.npz")
     ap.add_argument("--queries", type=int, default=2000, help="timed queries per index")
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--rounds", type=int, default=12, help="early-stopping rounds of the batch network")
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--scale", type=float, default=1.0, help="index size factor (for quick runs)")
     ap.add_argument("--reuse-data", default="", help="timings file of an earlier run: keep its features, M and "
@@ -232,13 +254,22 @@ def main(argv: List[str] = None):
     out = {"meta": {"trained_on": [list(x) for x in TRAIN_INDICES], "heldout": list(HELDOUT_INDEX),
                     "train_queries": int(tr.sum()), "heldout_queries": int(te.sum())}}
     kept = json.load(open(a.reuse_model)) if a.reuse_model else None
+    # a seeded tenth of the training queries validates the batch network
+    # (early stopping on validation regret); the held-out index stays unseen
+    rng = np.random.default_rng(0xADAB04)
+    va = tr & (rng.random(tr.size) < 0.1)
+    fit = tr & ~va
     for slot, name, T in ((0, "single", T1), (1, "batch", TB)):
         if kept is not None and name == "single":
             out[name] = kept[name]
             print(name, "kept from", a.reuse_model, flush=True)
             continue
-        tau = fit_threshold(M[tr], R[tr], T[tr])
-        learned, curve = _train(ctx, slot, F, T, tr, a.epochs)
+        if slot == 1:
+            tau = fit_threshold(M[fit], R[fit], T[fit])
+            learned, curve, val_r = _train(ctx, slot, F, T, fit, a.epochs, va=va, rounds=a.rounds)
+        else:
+            tau = fit_threshold(M[tr], R[tr], T[tr])
+            learned, curve, val_r = _train(ctx, slot, F, T, tr, a.epochs)
         thr = M.astype(np.float64) >= tau * R.astype(np.float64)
         res = {
             "tau": tau,
@@ -246,8 +277,12 @@ def main(argv: List[str] = None):
             "heldout_regret_threshold": regret(thr[te], T[te]), "heldout_regret_learned": regret(learned[te], T[te]),
             "first_loss": float(curve[0]), "final_loss": float(curve[-1]),
             "labels": "log ms; " + ("single-query synchronized wall time" if slot == 0 else
-                                    "device time of the query's share of a batch"),
+                                    "device time of the query's share of a batch of 256"),
         }
+        if val_r is not None:
+            res["validation_regret_learned"] = val_r
+            res["validation_regret_threshold"] = regret(thr[va], T[va])
+            res["validation_queries"] = int(va.sum())
         print(name, json.dumps(res), flush=True)
         out[name] = dict(ctx.model_dict(slot), meta=res)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
