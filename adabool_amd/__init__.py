@@ -592,9 +592,10 @@ class Context:
 
     def autotune(self, d_keys, qoff, path: int = AB_PATH_AUTO, repeats: int = 3) -> list:
         """Times the launch-parameter grid on this batch and keeps the fastest
-        setting; returns every candidate, fastest first."""
+        setting whose bitmaps equal the entry settings'; returns every
+        candidate, valid ones (mismatched_words == 0) first, fastest first."""
         off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
-        cap = 64
+        cap = 128  # the grid holds 72 candidates
         buf = (_TuneConfig * cap)()
         n = C.c_uint32()
         _check(_lib.ab_ctx_autotune(self._h, _ptr(d_keys), _ptr(off), off.size - 1, path, repeats, buf, cap,
@@ -750,7 +751,7 @@ class Context:
 # ---------------------------------------------------------------------------
 class _TuneConfig(C.Structure):
     _fields_ = [("group", C.c_uint32), ("dense_threads", C.c_uint32), ("sparse_chunk", C.c_uint32),
-                ("dense_columns", C.c_int), ("ms", C.c_double)]
+                ("dense_columns", C.c_int), ("ms", C.c_double), ("mismatched_words", C.c_uint64)]
 
 
 class _ServerConfig(C.Structure):

@@ -60,8 +60,9 @@ def tune(index: Index, keys: np.ndarray, qoff: np.ndarray, batch_sizes, model: s
             torch.zeros(1, dtype=torch.int32, device="cuda")
         torch.cuda.synchronize()
         res = ctx.autotune(dk, sub, AB_PATH_AUTO, repeats)
-        best, worst = res[0], res[-1]
-        out[str(b)] = dict(best, candidates=len(res), worst_ms=worst["ms"],
+        valid = [r for r in res if not r["mismatched_words"]]
+        best, worst = valid[0], valid[-1]
+        out[str(b)] = dict(best, candidates=len(res), mismatched_candidates=len(res) - len(valid), worst_ms=worst["ms"],
                            default_ms=next((r["ms"] for r in res if r["group"] == 64 and r["dense_threads"] == 512
                                             and r["sparse_chunk"] == 4096 and r["dense_columns"]), None))
         log(f"[autotune] batch {b}: best {best} ({len(res)} candidates, worst {worst['ms']:.3f} ms)")

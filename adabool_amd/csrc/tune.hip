@@ -1,5 +1,6 @@
 // tune.hip -- launch-parameter autotuner: times every candidate setting of a
-// context on a sample batch and keeps the fastest.
+// context on a sample batch and keeps the fastest one whose result bitmaps
+// equal those of the settings the context held on entry.
 //
 // Knobs (all on the device-dispatched batch path):
 //   dense group width  queries that share one pass over the value-tiled key
@@ -22,6 +23,33 @@
 
 using namespace ab;
 
+namespace {
+
+// Words that differ between two bitmap sets of n4 uint4 each; one atomic per wave.
+__global__ void __launch_bounds__(256) k_count_diff(const uint4 *__restrict__ a, const uint4 *__restrict__ b,
+                                                    uint64_t n4, unsigned long long *__restrict__ out)
+{
+    uint32_t d = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 x = a[i], y = b[i];
+        d += (x.x != y.x) + (x.y != y.y) + (x.z != y.z) + (x.w != y.w);
+    }
+    for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+    if ((threadIdx.x & 63) == 0 && d) atomicAdd(out, (unsigned long long)d);
+}
+
+struct TuneScratch {
+    void *ref = nullptr, *cand = nullptr, *diff = nullptr;
+    ~TuneScratch()
+    {
+        if (ref) (void)hipFree(ref);
+        if (cand) (void)hipFree(cand);
+        if (diff) (void)hipFree(diff);
+    }
+};
+
+}  // namespace
+
 extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq,
                                      int path, uint32_t repeats, ab_tune_config *out, uint32_t cap,
                                      uint32_t *n_out)
@@ -41,7 +69,20 @@ extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const ui
         return AB_DEVICE_ERROR;
     }
     const uint32_t saved[4] = {c->group, c->dense_threads, c->sparse_chunk, (uint32_t)c->dense_columns};
-    ab_status st = AB_OK;
+    // result bitmaps of the entry settings: every candidate must reproduce them
+    const uint64_t bm_bytes = (uint64_t)nq * c->ix->stride;  // stride: a multiple of 16 bytes
+    TuneScratch sc;
+    if (hipMalloc(&sc.ref, bm_bytes) != hipSuccess || hipMalloc(&sc.cand, bm_bytes) != hipSuccess ||
+        hipMalloc(&sc.diff, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipGetLastError();
+        set_error("ab_ctx_autotune: out of device memory for the result check (" + std::to_string(2 * bm_bytes) +
+                  " bytes)");
+        return AB_OOM;
+    }
+    ab_status st = ab_query_batch(c, d_keys, h_qoff, nq, path, (uint32_t *)sc.ref, nullptr, 0, nullptr, nullptr);
+    const uint32_t diff_grid = (uint32_t)std::min<uint64_t>(4096, (bm_bytes / 16 + 255) / 256);
     for (uint32_t g : groups)
         for (uint32_t t : threads)
             for (uint32_t ch : chunks)
@@ -51,9 +92,25 @@ extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const ui
                     if ((st = ab_ctx_set_dense_threads(c, t)) != AB_OK) break;
                     if ((st = ab_ctx_set_sparse_chunk(c, ch)) != AB_OK) break;
                     if ((st = ab_ctx_set_dense_columns(c, col)) != AB_OK) break;
-                    // warm-up (allocation, graph capture), then timed repeats
-                    st = ab_query_batch(c, d_keys, h_qoff, nq, path, nullptr, nullptr, 0, nullptr, nullptr);
+                    // checked warm-up (allocation, graph capture), then timed repeats
+                    st = ab_query_batch(c, d_keys, h_qoff, nq, path, (uint32_t *)sc.cand, nullptr, 0, nullptr,
+                                        nullptr);
                     if (st != AB_OK) break;
+                    unsigned long long diff = 0;
+                    if (hipMemsetAsync(sc.diff, 0, sizeof(diff), c->stream) != hipSuccess) st = AB_DEVICE_ERROR;
+                    if (st == AB_OK) {
+                        k_count_diff<<<diff_grid, 256, 0, c->stream>>>((const uint4 *)sc.ref, (const uint4 *)sc.cand,
+                                                                       bm_bytes / 16, (unsigned long long *)sc.diff);
+                        if (hipGetLastError() != hipSuccess ||
+                            hipMemcpyAsync(&diff, sc.diff, sizeof(diff), hipMemcpyDeviceToHost, c->stream) !=
+                                hipSuccess ||
+                            hipStreamSynchronize(c->stream) != hipSuccess)
+                            st = AB_DEVICE_ERROR;
+                    }
+                    if (st != AB_OK) {
+                        set_error("ab_ctx_autotune: result check failed");
+                        break;
+                    }
                     (void)hipEventRecord(e0, c->stream);
                     for (uint32_t r = 0; r < repeats && st == AB_OK; ++r)
                         st = ab_query_batch(c, d_keys, h_qoff, nq, path, nullptr, nullptr, 0, nullptr, nullptr);
@@ -69,6 +126,7 @@ extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const ui
                     cfg.sparse_chunk = ch;
                     cfg.dense_columns = col && col_fits(c, group_mode(cfg.group), cfg.group);
                     cfg.ms = (double)ms / repeats;
+                    cfg.mismatched_words = diff;
                     all.push_back(cfg);
                 }
     (void)hipEventDestroy(e0);
@@ -80,9 +138,19 @@ extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const ui
         (void)ab_ctx_set_dense_columns(c, (int)saved[3]);
         return st != AB_OK ? st : AB_DEVICE_ERROR;
     }
+    // valid candidates first, each part fastest first
     std::stable_sort(all.begin(), all.end(), [](const ab_tune_config &a, const ab_tune_config &b) {
+        if ((a.mismatched_words == 0) != (b.mismatched_words == 0)) return a.mismatched_words == 0;
         return a.ms < b.ms;
     });
+    if (all.front().mismatched_words) {  // no candidate reproduced the entry settings' bitmaps
+        (void)ab_ctx_set_dense_group(c, saved[0]);
+        (void)ab_ctx_set_dense_threads(c, saved[1]);
+        (void)ab_ctx_set_sparse_chunk(c, saved[2]);
+        (void)ab_ctx_set_dense_columns(c, (int)saved[3]);
+        set_error("ab_ctx_autotune: every candidate's bitmaps differ from the entry settings' result");
+        return AB_DEVICE_ERROR;
+    }
     const ab_tune_config &best = all.front();
     (void)ab_ctx_set_dense_group(c, best.group);
     (void)ab_ctx_set_dense_threads(c, best.dense_threads);

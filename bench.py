@@ -415,7 +415,8 @@ def main():
         res = base_ctx.autotune(d_keys, qo, path, repeats=2)
         dflt = next((r["ms"] for r in res if r["group"] == 64 and r["dense_threads"] == 512
                      and r["sparse_chunk"] == 4096 and r["dense_columns"]), None)
-        tuned = dict(res[0], default_ms=dflt, candidates=len(res), tile_shift=info["tile_shift"])
+        tuned = dict(res[0], default_ms=dflt, candidates=len(res), tile_shift=info["tile_shift"],
+                     mismatched_candidates=sum(r["mismatched_words"] > 0 for r in res))
         by_tile = {info["tile_shift"]: res[0]["ms"]}
         extra = [int(x) for x in args.tile_shifts.split(",") if x and int(x) != info["tile_shift"]]
         if extra:
@@ -425,6 +426,7 @@ def main():
                 ix2 = ab.Index.build_device(d_pk, d_pv, spec.nkeys, spec.nvalues, ts)
                 ctx2 = setup_ctx(ix2)
                 r2 = ctx2.autotune(d_keys, qo, path, repeats=2)
+                tuned["mismatched_candidates"] += sum(r["mismatched_words"] > 0 for r in r2)
                 by_tile[ts] = r2[0]["ms"]
                 if r2[0]["ms"] < tuned["ms"]:
                     if ctx is not base_ctx:
@@ -517,6 +519,15 @@ def main():
                             and bool((i[: int(counts[0])].cpu() == ids0).all()) for _, i, n in bufs[1:])
         log(f"batched ({dispatch}, mode {args.batch_mode}), {len(ctxs)} streams: {qps:.0f} q/s, "
             f"{wall / args.steps * 1e3:.2f} ms/step; outputs equal the serial run: {streams_equal}")
+    smp = None
+    if not args.no_cpu:
+        # host copies of the first queries' results of the timed run (the last
+        # context's buffers, written by nothing else), for the checker
+        ns = min(64, nq)
+        sb, si, _ = bufs[-1]
+        smp = (sb[: ns * sw].cpu().numpy().view(np.uint32).reshape(ns, sw),
+               np.concatenate([si[i * nv: i * nv + int(counts[i])].cpu().numpy().view(np.uint32) for i in range(ns)]),
+               counts[:ns].astype(np.uint32))
     ksteps = max(2, min(args.steps, 5))
     kt = kernel_times(path, ksteps)
     log("kernels: " + ", ".join(f"{k}={v[0] / ksteps:.3f}ms/{v[1] // ksteps}" for k, v in kt.items() if v[1]))
@@ -623,16 +634,10 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        # host copies of the first queries' GPU results of the timed run, for the checker
-        ns = min(64, nq)
-        smp_bm = d_bm[: ns * sw].cpu().numpy().view(np.uint32).reshape(ns, sw)
-        smp_ids = np.concatenate([d_ids[i * nv: i * nv + int(counts[i])].cpu().numpy().view(np.uint32)
-                                  for i in range(ns)])
         more = [("c3_mix",) + c3_data] if c3_data is not None else []
         t = time.time()
-        cpu = cpu_baseline(pk, pv, spec, qk, qo, counts, args.cpu_budget,
-                           sample=(smp_bm, smp_ids, counts[:ns].astype(np.uint32)), more=more)
-        del smp_bm, smp_ids
+        cpu = cpu_baseline(pk, pv, spec, qk, qo, counts, args.cpu_budget, sample=smp, more=more)
+        del smp
         log(f"cpu baseline: {cpu['value']:.1f} q/s on {cpu['cores']} threads ({time.time() - t:.1f}s), "
             f"counts match: {cpu['counts_match_gpu']}")
 

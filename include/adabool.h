@@ -188,8 +188,13 @@ ab_status ab_ctx_set_batch_mode(ab_ctx *ctx, int mode);
  *   dense group width {8,16,32,48,64,96,128,192,256} x dense threads {256,512}
  *   x sparse chunk {1024,4096} x dense columns {on,off},
  * `repeats` calls each after one warm-up call, and leaves the context set to
- * the fastest.  out (may be NULL) receives up to cap candidates, fastest
- * first, with their mean ms per batch; n_out the number timed.  The value
+ * the fastest valid one.  Each warm-up call's result bitmaps are compared with
+ * those of the settings the context held on entry (2 x nq bitmaps of device
+ * scratch, freed on return); a candidate that differs is reported with its
+ * mismatched word count and never chosen, and if none matches the entry
+ * settings are kept and AB_DEVICE_ERROR returned.  out (may be NULL) receives
+ * up to cap candidates, valid ones first, each part fastest first, with their
+ * mean ms per batch; n_out the number timed.  The value
  * tile is an index layout parameter (ab_index_build's tile_shift): tune it
  * by building one index per candidate.  No reference counterpart (its launch
  * shapes are compile-time template arguments). */
@@ -199,6 +204,7 @@ typedef struct ab_tune_config {
     uint32_t sparse_chunk;
     int dense_columns;       /* effective (on only where the LDS fits) */
     double ms;               /* mean device ms per batch */
+    uint64_t mismatched_words; /* result-bitmap words that differ from the entry settings' (0: valid) */
 } ab_tune_config;
 ab_status ab_ctx_autotune(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq, int path,
                           uint32_t repeats, ab_tune_config *out, uint32_t cap, uint32_t *n_out);

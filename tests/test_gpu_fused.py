@@ -218,3 +218,34 @@ def test_host_dispatch_caps_group_width(c1):
     res = ctx.query_batch(qk, qo, ab.AB_PATH_DENSE)
     _verify(csr, qk, qo, res, "host dispatch, group 128")
     ctx.close()
+
+
+def test_autotune_winner_matches_oracle(c1):
+    """ab_ctx_autotune on a small C1 batch in graph mode: every candidate of
+    the grid reproduces the entry settings' bitmaps, the context is left on
+    the winner, and the winner's results (left in place, and applied to a
+    fresh context) equal the oracle's."""
+    import torch
+
+    ix, csr = c1
+    qk, qo = _mixed(0xADAB0207, 40, configs.C1.nkeys)
+    d_keys = torch.from_numpy(qk.view(np.int32)).cuda()
+    ctx = ab.Context(ix)
+    ctx.set_batch_mode(ab.Context.BATCH_GRAPH)
+    ctx.set_batch_threshold(0.01)
+    print("autotune on 40 C1 queries", flush=True)
+    res = ctx.autotune(d_keys, qo, ab.AB_PATH_THRESHOLD, repeats=1)
+    assert len(res) == 9 * 2 * 2 * 2
+    assert all(r["mismatched_words"] == 0 for r in res), [r for r in res if r["mismatched_words"]][:4]
+    assert all(r["ms"] > 0 for r in res) and [r["ms"] for r in res] == sorted(r["ms"] for r in res)
+    best = res[0]
+    out = ctx.query_batch(qk, qo, ab.AB_PATH_THRESHOLD)
+    assert (out[3] == ab.AB_PATH_DENSE).any() and (out[3] == ab.AB_PATH_SPARSE).any(), out[3]
+    _verify(csr, qk, qo, out, f"autotuned {best}")
+    ctx2 = ab.Context(ix)
+    ctx2.set_batch_mode(ab.Context.BATCH_GRAPH)
+    ctx2.set_batch_threshold(0.01)
+    ctx2.apply_tuning(best)
+    _verify(csr, qk, qo, ctx2.query_batch(qk, qo, ab.AB_PATH_THRESHOLD), f"applied {best}")
+    ctx2.close()
+    ctx.close()

@@ -1,9 +1,10 @@
 """Full-size workloads against the oracle, every query checked.
 
-* C2 at the timed configuration: the 4M x 1M filter device-built at value
-  tile 2^9, 256-query dense groups with column masks, 256-thread work-groups,
-  learned dispatch, graph mode; three calls of the same 1024-query batch (the
-  first runs as launches, the second is captured, the third replays).
+* C2 at the timed configurations: the 4M x 1M filter device-built at value
+  tile 2^10 with 512-thread work-groups and at 2^9 with 256, 256-query dense
+  groups with column masks, learned and threshold dispatch, graph mode; three
+  calls of the same 1024-query batch (the first runs as launches, the second
+  is captured, the third replays).
 * C3 on the C2 filter: power-law batches of 1, 64 and 4096 queries in graph
   mode, with both paths firing.
 * 65,536 C3 queries in ONE device-dispatched batch (above a grid's y limit of
@@ -113,10 +114,9 @@ def c2():
     dk = torch.from_numpy(k.view(np.int32)).cuda()
     dv = torch.from_numpy(v.view(np.int32)).cuda()
     ix = ab.Index.build_device(dk, dv, configs.C2.nkeys, configs.C2.nvalues, 9)
-    del dk, dv
-    torch.cuda.empty_cache()
-    yield ix, csr
+    yield ix, csr, (dk, dv)
     ix.close()
+    del dk, dv
     torch.cuda.empty_cache()
 
 
@@ -133,23 +133,31 @@ def _ctx(ix, group, threads, columns=True):
     return ctx
 
 
-def test_c2_at_the_timed_configuration(c2):
-    """The batch the benchmark times, with its settings: every query's
-    count, bitmap and ids equal the oracle's on launch, capture and replay."""
-    ix, csr = c2
+@pytest.mark.parametrize("tile,threads", [(10, 512), (9, 256)])
+def test_c2_at_the_timed_configuration(c2, tile, threads):
+    """The batch the benchmark times, with the settings its autotuner picks
+    (tile 2^10, 512 threads, 256-query groups; 2^9 and 256 threads before):
+    every query's count, bitmap and ids equal the oracle's on launch, capture
+    and replay."""
+    ix, csr, (dk, dv) = c2
     qk, qo = configs.make_queries(configs.C2, configs.C2_Q)
     want = csr.union_batch_digest(qk, qo, NT)
-    ctx = _ctx(ix, 256, 256)
+    tix = ix if tile == 9 else ab.Index.build_device(dk, dv, configs.C2.nkeys, configs.C2.nvalues, tile)
+    assert tix.info["tile_shift"] == tile
+    ctx = _ctx(tix, 256, threads)
     for path in (ab.AB_PATH_AUTO, ab.AB_PATH_THRESHOLD):
-        _run_checked(ctx, qk, qo, path, want, calls=3, ids_on=(0, 2), bitmaps_on=(0, 1, 2), what=f"C2 path {path}")
+        _run_checked(ctx, qk, qo, path, want, calls=3, ids_on=(0, 2), bitmaps_on=(0, 1, 2),
+                     what=f"C2 tile {tile} threads {threads} path {path}")
     ctx.close()
+    if tix is not ix:
+        tix.close()
 
 
 @pytest.mark.parametrize("batch", [1, 64, 4096])
 def test_c3_batches_on_c2_filter(c2, batch):
     """C3 power-law queries (1..50000 Zipf-weighted keys) in graph mode at
     batch sizes 1, 64 and 4096; batches of one run as separate calls."""
-    ix, csr = c2
+    ix, csr, _ = c2
     qk, qo = configs.make_queries(configs.C2, configs.C3_Q, 4096)
     ctx = _ctx(ix, 256 if batch >= 1024 else 32, 256)
     if batch == 1:
