@@ -429,21 +429,6 @@ template <class F> __device__ __forceinline__ void for_bits(const M256 &m, F &&f
     for_bits(m.w[2], 128, f);
     for_bits(m.w[3], 192, f);
 }
-__device__ __forceinline__ void or_mask(uint32_t *p, uint32_t m) { atomicOr(p, m); }
-__device__ __forceinline__ void or_mask(unsigned long long *p, unsigned long long m) { atomicOr(p, m); }
-// Both halves unconditionally: lanes of a wave hold different masks, and a
-// test per half would split the wave into two partly masked atomics anyway
-// (ORing a zero half is harmless); 5% less union time than testing.
-__device__ __forceinline__ void or_mask(M128 *p, const M128 &m)
-{
-    atomicOr(&p->lo, m.lo);
-    atomicOr(&p->hi, m.hi);
-}
-__device__ __forceinline__ void or_mask(M256 *p, const M256 &m)
-{
-#pragma unroll
-    for (int i = 0; i < 4; ++i) atomicOr(&p->w[i], m.w[i]);
-}
 // 64 x 64 bit transpose across a wave: lane l holds row l (bit c = M[l][c]);
 // afterwards lane b holds column b (bit l = M[l][b]).  Six butterfly stages,
 // each swapping the off-diagonal s x s blocks between lanes l and l ^ s; the
@@ -494,6 +479,46 @@ __device__ __forceinline__ unsigned long long transpose64(unsigned long long x, 
         x = (lane & s) ? ((x & ~kLow[k]) | ((y & ~kLow[k]) >> s)) : ((x & kLow[k]) | ((y & kLow[k]) << s));
     }
     return x;
+}
+// Column masks in LDS (one query set per value of the tile, tv values): a
+// mask of up to 64 bits is one word per value; wider ones are kept as planes
+// of u64 words (word c of value v at plane c, index c * tv + v), so that the
+// lanes of a wave ORing into different values hit 32 bank pairs instead of
+// the 8 that a 16- or 32-byte stride per value leaves them.
+__device__ __forceinline__ void col_or(uint32_t *col, uint32_t v, uint32_t, uint32_t m) { atomicOr(&col[v], m); }
+__device__ __forceinline__ void col_or(unsigned long long *col, uint32_t v, uint32_t, unsigned long long m)
+{
+    atomicOr(&col[v], m);
+}
+// both halves (or all four words) unconditionally: lanes of a wave hold
+// different masks, and a test per word would split the wave into partly
+// masked atomics anyway (ORing a zero word is harmless; 5% less union time
+// than testing each half)
+__device__ __forceinline__ void col_or(M128 *col, uint32_t v, uint32_t tv, const M128 &m)
+{
+    unsigned long long *cp = reinterpret_cast<unsigned long long *>(col);
+    atomicOr(&cp[v], m.lo);
+    atomicOr(&cp[tv + v], m.hi);
+}
+__device__ __forceinline__ void col_or(M256 *col, uint32_t v, uint32_t tv, const M256 &m)
+{
+    unsigned long long *cp = reinterpret_cast<unsigned long long *>(col);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) atomicOr(&cp[i * tv + v], m.w[i]);
+}
+// 64-query chunk c of the column mask of value v
+__device__ __forceinline__ unsigned long long col_word(const uint32_t *col, uint32_t v, uint32_t, uint32_t)
+{
+    return col[v];
+}
+__device__ __forceinline__ unsigned long long col_word(const unsigned long long *col, uint32_t v, uint32_t, uint32_t)
+{
+    return col[v];
+}
+template <class M>
+__device__ __forceinline__ unsigned long long col_word(const M *col, uint32_t v, uint32_t tv, uint32_t c)
+{
+    return reinterpret_cast<const unsigned long long *>(col)[c * tv + v];
 }
 // 64-query chunk c of a mask
 __device__ __forceinline__ unsigned long long mask_word(uint32_t m, uint32_t) { return m; }
@@ -561,7 +586,7 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
             for_bits(m, [&](uint32_t b) { atomicOr(&tiles[b * tw + lw], bits); });
         }
     };
-    auto put = [&](uint32_t v) { or_mask(&col[v & vmask], m); };
+    auto put = [&](uint32_t v) { col_or(col, v & vmask, vmask + 1, m); };
     const bool use_col = MODE != kModeBitmap && col != nullptr;
     if constexpr (T == AB_BITSET) {
         const uint32_t base = h >> 5;
@@ -780,10 +805,9 @@ __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, co
         // for every query of the group (zeros included).
         const bool pair = ((out_stride & 1) == 0) && ((reinterpret_cast<uintptr_t>(out) & 7) == 0);
         for (uint32_t gi = wave; gi < (tv >> 6); gi += NW) {
-            const M cv = col[gi * 64 + lane];
             const uint32_t w = 2 * gi;  // first of the two words, within the tile's row
             for (uint32_t c = 0; c * 64 < B; ++c) {
-                const unsigned long long qv = mask_word(cv, c);
+                const unsigned long long qv = col_word(col, gi * 64 + lane, tv, c);
                 const uint32_t nb = B - c * 64 < 64 ? B - c * 64 : 64;
                 unsigned long long mine = 0;
                 if (__ballot(qv != 0) != 0) mine = transpose64(qv, lane);
