@@ -9,7 +9,8 @@
 //                      u64 for more than 64 queries)
 //   dense threads      work-group size of the persistent kernel (256 / 512),
 //                      i.e. waves per value tile
-//   sparse chunk       records per sparse work item (256..8192)
+//   sparse chunk       records per sparse route chunk (4096 / 8192; larger
+//                      chunks give longer per-slice runs of references)
 //   dense columns      per-value query masks in LDS (on / off)
 // The value tile itself is a layout parameter of the index (ab_index_build's
 // tile_shift); tuning it means building the index once per candidate.
@@ -60,7 +61,7 @@ extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const ui
     }
     const uint32_t groups[] = {8, 16, 32, 48, 64, 96, 128, 192, 256};
     const uint32_t threads[] = {256, 512};
-    const uint32_t chunks[] = {1024, 4096};
+    const uint32_t chunks[] = {4096, 8192};
     const int columns[] = {1, 0};
     std::vector<ab_tune_config> all;
     hipEvent_t e0 = nullptr, e1 = nullptr;

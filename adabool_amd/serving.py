@@ -46,6 +46,8 @@ def main(argv=None):
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     ap.add_argument("--tile-shifts", default="10", help="value tiles the autotuner also tries (device builds)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--settings", default="",
+                    help="group/threads/chunk/tile used as given instead of autotuning, e.g. 128/512/8192/10")
     a = ap.parse_args(argv)
 
     import torch
@@ -79,7 +81,14 @@ def main(argv=None):
     # ---- autotune on a sample batch: launch parameters, then other value tiles
     tuned = None
     serve_ix = ix
-    if not a.no_autotune:
+    if a.settings:
+        g, t, ch, ts = (int(x) for x in a.settings.split("/"))
+        tuned = {"group": g, "dense_threads": t, "sparse_chunk": ch, "dense_columns": True, "tile_shift": ts,
+                 "given": True}
+        if ts != info["tile_shift"]:
+            serve_ix = Index.build_device(dk, dv, spec.nkeys, spec.nvalues, ts)
+        log(f"settings as given: {tuned}")
+    elif not a.no_autotune:
         ns = min(nq, 256)
         sk = torch.from_numpy(np.ascontiguousarray(qk[: int(qo[ns])]).view(np.int32)).cuda()
         so = qo[: ns + 1]

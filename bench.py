@@ -270,10 +270,10 @@ def c5_leg(ab, torch, model_path: str, nq: int = 8192, rates=(0.0, 10000.0, 5000
     try:
         if os.path.exists(model_path):
             srv.load_model(model_path)
-        for c in srv.contexts:  # the serving autotune's choice for this filter
+        for c in srv.contexts:  # the best of a sweep of these settings on this filter
             c.set_dense_group(128)
             c.set_dense_threads(512)
-            c.set_sparse_chunk(1024)
+            c.set_sparse_chunk(8192)
         srv.replay(qk[: int(qo[2048])], qo[:2049], 0.0, 7)
         srv.stats(reset=True)
         ref = None

@@ -186,7 +186,7 @@ ab_status ab_ctx_set_batch_mode(ab_ctx *ctx, int mode);
 /* Autotuner.  Times ab_query_batch on the given batch (device keys, host
  * offsets; union only, no compaction) for every candidate of the grid
  *   dense group width {8,16,32,48,64,96,128,192,256} x dense threads {256,512}
- *   x sparse chunk {1024,4096} x dense columns {on,off},
+ *   x sparse chunk {4096,8192} x dense columns {on,off},
  * `repeats` calls each after one warm-up call, and leaves the context set to
  * the fastest valid one.  Each warm-up call's result bitmaps are compared with
  * those of the settings the context held on entry (2 x nq bitmaps of device
