@@ -62,7 +62,8 @@ struct Plan {
     uint32_t grid;  // work-groups of the persistent launch
     bool packed;    // per-query id offsets follow qoff in the staged block
     size_t stage_bytes;
-    uint64_t mask_stride, any_stride;  // elements / words per group
+    uint64_t mask_stride, any_stride;  // elements / u32 rank words per group
+    uint32_t ngroups;                  // dense groups the batch can have
     uint32_t *bm;
     uint64_t stride, words;
 };
@@ -133,7 +134,10 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     {
         TimedSpan ts(c, AB_K_DENSE_MASK);
         launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
-                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 0, s);
+                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskAny, s);
+        launch_mask_rank((uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, p.ngroups, (const uint32_t *)c->plan.p, s);
+        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
+                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskSet, s);
     }
     if (p.Kcap) {
         {
@@ -176,7 +180,8 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     {
         TimedSpan ts(c, AB_K_DENSE_MASK);
         launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
-                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, 1, s);
+                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskClear, s);
+        launch_fill_zero(c->ganybm.p, (size_t)p.ngroups * p.any_stride * 4, s);
     }
     if (a.d_ids || a.d_counts) {
         {
@@ -218,9 +223,10 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     p.words = a.out_words;
     const uint32_t nq = a.nq;
     const uint32_t ngmax = (nq + p.G - 1) / p.G;
+    p.ngroups = ngmax;
     const uint64_t nkeys = ix->nkeys;
     p.mask_stride = (nkeys + 63) / 64 * 64;
-    p.any_stride = (nkeys + 31) / 32 + 16;
+    p.any_stride = mask_rank_words(nkeys);
     const size_t esize = p.mode == kModeMask32 ? 4 : (p.mode == kModeMask64 ? 8 : (p.mode == kModeMask128 ? 16 : 32));
 
     // write-once sparse: slices of 2^18 values; refs hold type << 29 | index

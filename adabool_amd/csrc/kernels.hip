@@ -552,6 +552,16 @@ __device__ __forceinline__ typename MaskOf<MODE>::T probe(const void *__restrict
         return hit ? reinterpret_cast<const unsigned long long *>(m)[k] : 0ull;
 }
 
+// Rank words of device-dispatched groups (one uint4 per 64 keys: any-query
+// bits 0-31 and 32-63, rank of the first key): the u32 holding key k's bit,
+// and key k's rank among the group's keys.
+__device__ __forceinline__ uint32_t rank_word_index(uint32_t k) { return (k >> 6) * 4 + ((k >> 5) & 1); }
+__device__ __forceinline__ uint32_t key_rank(const uint4 &w, uint32_t k)
+{
+    const unsigned long long bits = ((unsigned long long)w.y << 32) | w.x;
+    return w.z + (uint32_t)__popcll(bits & ((1ull << (k & 63)) - 1));
+}
+
 // Picks w[j] for a run-time j < N without dynamic register indexing.
 template <int N>
 __device__ __forceinline__ uint32_t pick(const uint32_t (&w)[8], uint32_t j)
@@ -664,7 +674,7 @@ __device__ __forceinline__ void dense_emit(uint32_t h, const uint32_t (&w)[8], t
 // together (the any-bit is exactly "mask != 0"); then the RS emits -- unrolled
 // for ARRAY1, whose emit is a few instructions, else taken from the front of a
 // rotating register set (no run-time register indexing, so no scratch).
-template <int MODE, int T>
+template <int MODE, int T, bool RANK>
 __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *kb, uint32_t nv, uint32_t rec_base,
                                             const void *__restrict__ mask, const uint32_t *__restrict__ anybm,
                                             uint32_t lane, uint32_t *tiles, uint32_t tw, uint32_t tile_word0,
@@ -683,7 +693,15 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
         for (int j = 0; j < RS; ++j) {
             const uint32_t r = (s0 + j) * 64 + lane;
             key[j] = r < nv ? kb[r] : 0u;
-            hit[j] = r < nv ? (anybm[key[j] >> 5] >> (key[j] & 31)) & 1u : 0u;
+            if constexpr (RANK) {
+                // one 16-byte load: the key's any-query bit and its rank in the
+                // group's mask table (key[j] becomes the mask index)
+                const uint4 rw = reinterpret_cast<const uint4 *>(anybm)[key[j] >> 6];
+                hit[j] = r < nv ? (((key[j] & 32) ? rw.y : rw.x) >> (key[j] & 31)) & 1u : 0u;
+                key[j] = key_rank(rw, key[j]);
+            } else {
+                hit[j] = r < nv ? (anybm[key[j] >> 5] >> (key[j] & 31)) & 1u : 0u;
+            }
             any_hit |= hit[j];
         }
         if (!any_hit) continue;
@@ -729,7 +747,7 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
 // One value tile for a group of B queries (output slots qids[0..B)).
 // lds: per-wave key staging (1024 u32 per wave), B query row tiles, and for
 // query groups with use_col one column mask per value of the tile.
-template <int MODE, int NT>
+template <int MODE, int NT, bool RANK = false>
 __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, const void *__restrict__ mask,
                                            const uint32_t *__restrict__ anybm, uint32_t B,
                                            const uint32_t *qids, uint32_t *__restrict__ out, uint64_t out_stride,
@@ -783,12 +801,12 @@ __device__ __forceinline__ void dense_tile(const DevIndex &ix, uint32_t tile, co
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         switch (type) {
-        case AB_ARRAY1: dense_block<MODE, AB_ARRAY1>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
-        case AB_ARRAY4: dense_block<MODE, AB_ARRAY4>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
-        case AB_ARRAY8: dense_block<MODE, AB_ARRAY8>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
-        case AB_ARRAY16: dense_block<MODE, AB_ARRAY16>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
-        case AB_ARRAY32: dense_block<MODE, AB_ARRAY32>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
-        default: dense_block<MODE, AB_BITSET>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY1: dense_block<MODE, AB_ARRAY1, RANK>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY4: dense_block<MODE, AB_ARRAY4, RANK>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY8: dense_block<MODE, AB_ARRAY8, RANK>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY16: dense_block<MODE, AB_ARRAY16, RANK>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        case AB_ARRAY32: dense_block<MODE, AB_ARRAY32, RANK>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
+        default: dense_block<MODE, AB_BITSET, RANK>(ix, kb, nv, mt.w, mask, anybm, lane, tiles, tw, tile_word0, col, vmask); break;
         }
         // the staging area is rewritten by the next block of this wave
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -996,12 +1014,22 @@ void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uin
     }
 }
 
+// Group query masks of device-dispatched batches, compacted by rank.  Per
+// group, the keys its queries hold are numbered in key order (their rank);
+// the masks are stored at masks[rank], so that a group's table holds only
+// the keys it uses and stays small enough for the L2 cache (a table indexed
+// by key is 32 bytes x nkeys per 256-query group).  The rank words hold, per
+// 64 keys, the any-query bits (two u32) and the rank of the first of them,
+// so that one 16-byte load gives both the probe and the rank.
+
+// op kMaskAny: set the keys' any-query bits; kMaskSet: OR each query's bit
+// into its keys' masks (after k_mask_rank); kMaskClear: zero those masks.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict__ keys,
                                                      const uint64_t *__restrict__ qoff,
                                                      const uint32_t *__restrict__ qgroup, void *masks,
                                                      uint64_t mask_stride, uint32_t *anybms, uint64_t any_stride,
-                                                     uint32_t nkeys, int clear)
+                                                     uint32_t nkeys, int op)
 {
     using M = typename MaskOf<MODE>::T;
     const uint32_t q = blockIdx.y;
@@ -1010,23 +1038,26 @@ __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict_
     const uint32_t g = qg / kGroupSlots, b = qg % kGroupSlots;
     M *mask = reinterpret_cast<M *>(masks) + (uint64_t)g * mask_stride;
     uint32_t *any = anybms + (uint64_t)g * any_stride;
+    const uint4 *rw = reinterpret_cast<const uint4 *>(any);
     const uint64_t a = qoff[q], e = qoff[q + 1];
     for (uint64_t j = a + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < e; j += (uint64_t)gridDim.x * 256) {
         const uint32_t k = keys[j];
         if (k >= nkeys) continue;
-        if (clear) {
-            any[k >> 5] = 0;
-            mask[k] = M{};
+        if (op == kMaskAny) {
+            atomicOr(&any[rank_word_index(k)], 1u << (k & 31));
         } else {
-            atomicOr(&any[k >> 5], 1u << (k & 31));
-            set_mask_bit(&mask[k], b);
+            const uint32_t r = key_rank(rw[k >> 6], k);
+            if (op == kMaskSet)
+                set_mask_bit(&mask[r], b);
+            else
+                mask[r] = M{};
         }
     }
 }
 
 void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, const uint32_t *qgroup, uint32_t nq,
                         void *masks, uint64_t mask_stride, uint32_t *anybms, uint64_t any_stride, uint32_t nkeys,
-                        int clear, hipStream_t s)
+                        int op, hipStream_t s)
 {
     for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
         // a slice's kernels read qoff[q], qoff[q + 1] and qgroup[q] relative to its first query
@@ -1036,17 +1067,50 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
         const uint32_t *qg = qgroup + q0;
         if (mode == kModeMask32)
             hipLaunchKernelGGL(k_mask_groups<kModeMask32>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, clear);
+                               anybms, any_stride, nkeys, op);
         else if (mode == kModeMask64)
             hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, clear);
+                               anybms, any_stride, nkeys, op);
         else if (mode == kModeMask128)
             hipLaunchKernelGGL(k_mask_groups<kModeMask128>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, clear);
+                               anybms, any_stride, nkeys, op);
         else
             hipLaunchKernelGGL(k_mask_groups<kModeMask256>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, clear);
+                               anybms, any_stride, nkeys, op);
     }
+}
+
+// Ranks of group g's first keys per 64: one 1024-thread work-group per group,
+// each thread a run of consecutive rank words (exclusive prefix of the bit
+// counts).  Groups past plan[0] hold no bits and are skipped.
+__global__ __launch_bounds__(1024) void k_mask_rank(uint32_t *anybms, uint64_t any_stride, uint32_t nw,
+                                                    const uint32_t *__restrict__ plan)
+{
+    __shared__ unsigned long long sh[16];
+    const uint32_t g = blockIdx.x;
+    if (g >= plan[0]) return;
+    uint4 *rw = reinterpret_cast<uint4 *>(anybms + (uint64_t)g * any_stride);
+    const uint32_t per = (nw + 1023) / 1024, w0 = threadIdx.x * per;
+    const uint32_t w1 = w0 + per < nw ? w0 + per : nw;
+    unsigned long long n = 0;
+    for (uint32_t w = w0; w < w1; ++w) {
+        const uint2 b = *reinterpret_cast<const uint2 *>(&rw[w]);
+        n += (uint32_t)(__popc(b.x) + __popc(b.y));
+    }
+    unsigned long long tot;
+    unsigned long long run = block_excl_scan(n, sh, &tot);
+    for (uint32_t w = w0; w < w1; ++w) {
+        const uint2 b = *reinterpret_cast<const uint2 *>(&rw[w]);
+        rw[w].z = (uint32_t)run;
+        run += (uint32_t)(__popc(b.x) + __popc(b.y));
+    }
+}
+
+void launch_mask_rank(uint32_t *anybms, uint64_t any_stride, uint32_t nkeys, uint32_t ngroups, const uint32_t *plan,
+                      hipStream_t s)
+{
+    if (!ngroups) return;
+    hipLaunchKernelGGL(k_mask_rank, dim3(ngroups), dim3(1024), 0, s, anybms, any_stride, (nkeys + 63) / 64, plan);
 }
 
 // ---------------------------------------------------------------------------
@@ -1220,7 +1284,7 @@ __global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
             // tile-major: the groups of one tile run close together in time,
             // so the tile's key stream and records are still cached for each
             const uint32_t tile = (uint32_t)(item / ngroups), g = (uint32_t)(item - (uint64_t)tile * ngroups);
-            dense_tile<MODE, NT>(ix, tile, reinterpret_cast<const M *>(a.masks) + (uint64_t)g * a.mask_stride,
+            dense_tile<MODE, NT, true>(ix, tile, reinterpret_cast<const M *>(a.masks) + (uint64_t)g * a.mask_stride,
                                  a.anybms + (uint64_t)g * a.any_stride, a.grp_n[g], a.grp_q + (uint64_t)g * kGroupSlots,
                                  a.out, a.out_stride, a.out_words, a.use_col, lds);
         } else if (item < nd + ne) {
