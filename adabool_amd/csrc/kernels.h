@@ -69,16 +69,16 @@ hipError_t launch_dense_tiles(int mode, uint32_t threads, const DevIndex &ix, co
 void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uint8_t *sel, uint32_t nq,
                         uint8_t match, hipStream_t s);
 // Group masks are indexed by the key's rank among the group's keys.  Per
-// group, any_stride u32: the rank words (a uint4 per 64 keys: any-query bits
-// 0-31 and 32-63, rank of the first key), then one u32 total per run of 64
-// rank words (used while the ranks are formed).  A batch runs kMaskAny,
+// group, any_stride u32: the rank words (a uint2 per 32 keys: the any-query
+// bits, the rank of the first key), then one u32 total per run of 64 rank
+// words (used while the ranks are formed).  A batch runs kMaskAny,
 // launch_mask_rank, kMaskSet, the union, kMaskClear, and then zeroes the
 // rank words.
 constexpr int kMaskAny = 0, kMaskClear = 1, kMaskSet = 2;
 inline uint64_t mask_rank_words(uint32_t nkeys)
 {
-    const uint64_t nw = ((uint64_t)nkeys + 63) / 64;
-    return (nw + 1) * 4 + ((nw + 63) / 64 + 3) / 4 * 4;
+    const uint64_t nw = ((uint64_t)nkeys + 31) / 32;
+    return (nw + 1) * 2 + ((nw + 63) / 64 + 3) / 4 * 4;
 }
 void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, const uint32_t *qgroup,
                         uint32_t nq, void *masks, uint64_t mask_stride, uint32_t *anybms,

@@ -552,14 +552,13 @@ __device__ __forceinline__ typename MaskOf<MODE>::T probe(const void *__restrict
         return hit ? reinterpret_cast<const unsigned long long *>(m)[k] : 0ull;
 }
 
-// Rank words of device-dispatched groups (one uint4 per 64 keys: any-query
-// bits 0-31 and 32-63, rank of the first key): the u32 holding key k's bit,
+// Rank words of device-dispatched groups (one uint2 per 32 keys: the
+// any-query bits, the rank of the first key): the u32 holding key k's bit,
 // and key k's rank among the group's keys.
-__device__ __forceinline__ uint32_t rank_word_index(uint32_t k) { return (k >> 6) * 4 + ((k >> 5) & 1); }
-__device__ __forceinline__ uint32_t key_rank(const uint4 &w, uint32_t k)
+__device__ __forceinline__ uint32_t rank_word_index(uint32_t k) { return (k >> 5) * 2; }
+__device__ __forceinline__ uint32_t key_rank(const uint2 &w, uint32_t k)
 {
-    const unsigned long long bits = ((unsigned long long)w.y << 32) | w.x;
-    return w.z + (uint32_t)__popcll(bits & ((1ull << (k & 63)) - 1));
+    return w.y + (uint32_t)__popc(w.x & ((1u << (k & 31)) - 1));
 }
 
 // Picks w[j] for a run-time j < N without dynamic register indexing.
@@ -694,10 +693,10 @@ __device__ __forceinline__ void dense_block(const DevIndex &ix, const uint32_t *
             const uint32_t r = (s0 + j) * 64 + lane;
             key[j] = r < nv ? kb[r] : 0u;
             if constexpr (RANK) {
-                // one 16-byte load: the key's any-query bit and its rank in the
+                // one 8-byte load: the key's any-query bit and its rank in the
                 // group's mask table (key[j] becomes the mask index)
-                const uint4 rw = reinterpret_cast<const uint4 *>(anybm)[key[j] >> 6];
-                hit[j] = r < nv ? (((key[j] & 32) ? rw.y : rw.x) >> (key[j] & 31)) & 1u : 0u;
+                const uint2 rw = reinterpret_cast<const uint2 *>(anybm)[key[j] >> 5];
+                hit[j] = r < nv ? (rw.x >> (key[j] & 31)) & 1u : 0u;
                 key[j] = key_rank(rw, key[j]);
             } else {
                 hit[j] = r < nv ? (anybm[key[j] >> 5] >> (key[j] & 31)) & 1u : 0u;
@@ -1038,7 +1037,7 @@ __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict_
     const uint32_t g = qg / kGroupSlots, b = qg % kGroupSlots;
     M *mask = reinterpret_cast<M *>(masks) + (uint64_t)g * mask_stride;
     uint32_t *any = anybms + (uint64_t)g * any_stride;
-    const uint4 *rw = reinterpret_cast<const uint4 *>(any);
+    const uint2 *rw = reinterpret_cast<const uint2 *>(any);
     const uint64_t a = qoff[q], e = qoff[q + 1];
     for (uint64_t j = a + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < e; j += (uint64_t)gridDim.x * 256) {
         const uint32_t k = keys[j];
@@ -1046,7 +1045,7 @@ __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict_
         if (op == kMaskAny) {
             atomicOr(&any[rank_word_index(k)], 1u << (k & 31));
         } else {
-            const uint32_t r = key_rank(rw[k >> 6], k);
+            const uint32_t r = key_rank(rw[k >> 5], k);
             if (op == kMaskSet)
                 set_mask_bit(&mask[r], b);
             else
@@ -1080,7 +1079,7 @@ void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, co
     }
 }
 
-// Ranks of group g's first keys per 64, in two passes of one wave per 64 rank
+// Ranks of group g's first keys per 32, in two passes of one wave per 64 rank
 // words: the exclusive prefix inside the wave's run (and the run's total),
 // then the totals of the earlier runs added.  Groups past plan[0] hold no
 // bits and are skipped.
@@ -1090,16 +1089,12 @@ __global__ __launch_bounds__(64) void k_mask_rank_local(uint32_t *anybms, uint64
     const uint32_t g = blockIdx.y;
     if (g >= plan[0]) return;
     uint32_t *base = anybms + (uint64_t)g * any_stride;
-    uint4 *rw = reinterpret_cast<uint4 *>(base);
-    uint32_t *runtot = base + ((uint64_t)nw + 1) * 4;
+    uint2 *rw = reinterpret_cast<uint2 *>(base);
+    uint32_t *runtot = base + ((uint64_t)nw + 1) * 2;
     const uint32_t lane = lane_id(), w = blockIdx.x * 64 + lane;
-    uint32_t n = 0;
-    if (w < nw) {
-        const uint2 b = *reinterpret_cast<const uint2 *>(&rw[w]);
-        n = (uint32_t)(__popc(b.x) + __popc(b.y));
-    }
-    const uint32_t incl = wave_incl_scan(n);
-    if (w < nw) rw[w].z = incl - n;
+    const uint32_t n = w < nw ? (uint32_t)__popc(rw[w].x) : 0u;
+    const uint32_t incl = wave_incl_scan_dpp(n);
+    if (w < nw) rw[w].y = incl - n;
     if (lane == 63) runtot[blockIdx.x] = incl;
 }
 
@@ -1109,20 +1104,20 @@ __global__ __launch_bounds__(64) void k_mask_rank_add(uint32_t *anybms, uint64_t
     const uint32_t g = blockIdx.y;
     if (g >= plan[0] || blockIdx.x == 0) return;
     uint32_t *base = anybms + (uint64_t)g * any_stride;
-    uint4 *rw = reinterpret_cast<uint4 *>(base);
-    const uint32_t *runtot = base + ((uint64_t)nw + 1) * 4;
+    uint2 *rw = reinterpret_cast<uint2 *>(base);
+    const uint32_t *runtot = base + ((uint64_t)nw + 1) * 2;
     const uint32_t lane = lane_id(), w = blockIdx.x * 64 + lane;
     uint32_t pre = 0;
     for (uint32_t r = lane; r < blockIdx.x; r += 64) pre += runtot[r];
     pre = wave_sum(pre);
-    if (w < nw) rw[w].z += pre;
+    if (w < nw) rw[w].y += pre;
 }
 
 void launch_mask_rank(uint32_t *anybms, uint64_t any_stride, uint32_t nkeys, uint32_t ngroups, const uint32_t *plan,
                       hipStream_t s)
 {
     if (!ngroups) return;
-    const uint32_t nw = (nkeys + 63) / 64;
+    const uint32_t nw = (nkeys + 31) / 32;
     const dim3 grid((nw + 63) / 64, ngroups);
     hipLaunchKernelGGL(k_mask_rank_local, grid, dim3(64), 0, s, anybms, any_stride, nw, plan);
     hipLaunchKernelGGL(k_mask_rank_add, grid, dim3(64), 0, s, anybms, any_stride, nw, plan);
