@@ -1430,6 +1430,55 @@ __global__ __launch_bounds__(kScanNT) void k_chunk_scan(uint32_t *ccnt, uint32_t
     if (threadIdx.x == 0 && counts) counts[blockIdx.x] = (uint32_t)carry;
 }
 
+// One wave's 64-word piece of a bitmap: lane l holds word v, whose first
+// value is base; the piece's ids go to dst[off ...] in ascending order,
+// through the wave's staging area stg (kStage words).  Returns nothing; the
+// caller advances off by the piece's popcount.
+template <bool ALIGNED>
+__device__ __forceinline__ void expand_piece(uint32_t v, uint32_t base, uint64_t off, uint32_t *dst, uint32_t *stg,
+                                             uint32_t lane)
+{
+    const uint32_t cnt = __popc(v);
+    const uint32_t incl = wave_incl_scan_dpp(cnt);
+    const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    if (tot == 0) return;
+    const uint32_t sh = ALIGNED ? (uint32_t)(off & 3) : 0u;
+    uint32_t p = sh + incl - cnt;
+    while (v) {
+        stg[p++] = base + __builtin_ctz(v);
+        v &= v - 1;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if constexpr (ALIGNED) {
+        // staging index j <-> output index off - sh + j
+        uint32_t *d0 = dst + (off - sh);
+        const uint32_t end = sh + tot;
+        // whole quads [k0, k1); the partial first (sh > 0) and last (end % 4) quads by lanes 0 and 1
+        const uint32_t k0 = (sh + 3) >> 2, k1 = end >> 2;
+        if (lane == 0 && sh) {
+            for (uint32_t j = sh; j < 4 && j < end; ++j) d0[j] = stg[j];
+        }
+        if (lane == 1 && (end & 3) && (end >> 2) >= k0) {
+            for (uint32_t j = end & ~3u; j < end; ++j) d0[j] = stg[j];
+        }
+        uint32_t k = k0 + lane;
+        for (; k + 64 < k1; k += 128) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(stg + 4 * k);
+            const uint4 b = *reinterpret_cast<const uint4 *>(stg + 4 * (k + 64));
+            *reinterpret_cast<uint4 *>(d0 + 4 * k) = a;
+            *reinterpret_cast<uint4 *>(d0 + 4 * (k + 64)) = b;
+        }
+        if (k < k1) *reinterpret_cast<uint4 *>(d0 + 4 * k) = *reinterpret_cast<const uint4 *>(stg + 4 * k);
+    } else {
+        for (uint32_t j = lane; j < tot; j += 64) dst[off + j] = stg[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ccnt: per-chunk exclusive prefixes (k_chunk_scan).
 template <bool ALIGNED>
 __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
@@ -1469,46 +1518,7 @@ __global__ __launch_bounds__(256) void k_write_ids(const uint32_t *__restrict__ 
         uint32_t v = x[0];
 #pragma unroll
         for (int k = 1; k < 8; ++k) v = (i == k) ? x[k] : v;
-        const uint32_t cnt = __popc(v);
-        const uint32_t incl = wave_incl_scan_dpp(cnt);
-        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
-        if (tot == 0) continue;
-        const uint32_t sh = ALIGNED ? (uint32_t)(off & 3) : 0u;
-        uint32_t p = sh + incl - cnt;
-        const uint32_t base = (uint32_t)((ww0 + (uint64_t)i * 256 + lane) * 32);
-        while (v) {
-            stg[p++] = base + __builtin_ctz(v);
-            v &= v - 1;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if constexpr (ALIGNED) {
-            // staging index j <-> output index off - sh + j
-            uint32_t *d0 = dst + (off - sh);
-            const uint32_t end = sh + tot;
-            // whole quads [k0, k1); the partial first (sh > 0) and last (end % 4) quads by lanes 0 and 1
-            const uint32_t k0 = (sh + 3) >> 2, k1 = end >> 2;
-            if (lane == 0 && sh) {
-                for (uint32_t j = sh; j < 4 && j < end; ++j) d0[j] = stg[j];
-            }
-            if (lane == 1 && (end & 3) && (end >> 2) >= k0) {
-                for (uint32_t j = end & ~3u; j < end; ++j) d0[j] = stg[j];
-            }
-            uint32_t k = k0 + lane;
-            for (; k + 64 < k1; k += 128) {
-                const uint4 a = *reinterpret_cast<const uint4 *>(stg + 4 * k);
-                const uint4 b = *reinterpret_cast<const uint4 *>(stg + 4 * (k + 64));
-                *reinterpret_cast<uint4 *>(d0 + 4 * k) = a;
-                *reinterpret_cast<uint4 *>(d0 + 4 * (k + 64)) = b;
-            }
-            if (k < k1) *reinterpret_cast<uint4 *>(d0 + 4 * k) = *reinterpret_cast<const uint4 *>(stg + 4 * k);
-        } else {
-            for (uint32_t j = lane; j < tot; j += 64) dst[off + j] = stg[j];
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        expand_piece<ALIGNED>(v, (uint32_t)((ww0 + (uint64_t)i * 256 + lane) * 32), off, dst, stg, lane);
     }
 }
 

@@ -9,7 +9,9 @@
 * values next to 2^32 - 1 (nvalues = 2^32 - 1), where a bitmap would take
   512 MB per query, on host- and device-built indices;
 * the serving loop with ids_only: alone == in mix == oracle, and replay
-  counts equal the oracle's."""
+  counts equal the oracle's;
+* a filter of 1.06M values with a heavy key over the whole value range, at
+  aligned and unaligned offsets, in one part and in many."""
 import numpy as np
 import pytest
 
@@ -226,3 +228,41 @@ def test_server_ids_only_alone_equals_in_mix(c1):
         assert srv.stats()["arena_used_bytes"] == 0
     finally:
         srv.close()
+
+
+@pytest.fixture(scope="module")
+def wide():
+    """1.06M values (four value slices of 2^18 and a partial fifth) over 20k
+    keys, ~2.4M pairs, Zipf-weighted keys; a heavy key spans every slice."""
+    nvalues, nkeys = 4 * (1 << 18) + 12345, 20000
+    k, v = ab.gen_pairs(0xADAB0607, nvalues, nkeys, lo=1, hi=3, zipf_s=1.0)
+    hv = np.arange(0, nvalues, 4, dtype=np.uint32)
+    k = np.concatenate([k, np.full(hv.size, 17, np.uint32)])
+    v = np.concatenate([v, hv])
+    ix = ab.Index(k, v, nkeys, nvalues)
+    csr = oracle.CSR(k, v, nkeys, nvalues)
+    qk, qo = ab.gen_queries(0xADAB0608, nkeys, 120, size_dist=2, p1=1.5, lo=1, hi=6000, key_dist=1, zipf_s=1.0)
+    qs = [qk[qo[i]:qo[i + 1]] for i in range(120)]
+    qs[0] = np.array([17], np.uint32)
+    qs[1] = np.zeros(0, np.uint32)
+    qs[2] = np.arange(nkeys, dtype=np.uint32)
+    qs[3] = np.array([nkeys + 1, 17, 17, 3], np.uint32)
+    return ix, csr, qs
+
+
+@pytest.mark.parametrize("budget", [0, 60000])
+@pytest.mark.parametrize("aligned", [True, False])
+def test_ids_wide_filter(wide, aligned, budget):
+    ix, csr, qs = wide
+    ctx = ab.Context(ix)
+    if budget:
+        ctx.set_id_budget(budget)
+    print(f"ids wide filter aligned={aligned} budget={budget}", flush=True)
+    ids, counts, h, off, pos = _run_ids(ctx, qs, aligned)
+    _check_ranges(h, off, pos, counts, f"aligned={aligned} budget={budget}")
+    for i, q in enumerate(qs):
+        want = csr.union_ids(q)
+        assert counts[i] == want.size, f"q{i}"
+        assert np.array_equal(ids[i], want), f"q{i} ids"
+    assert counts[2] > 1 << 20  # the all-key query spans every slice
+    ctx.close()
