@@ -131,18 +131,24 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
         TimedSpan ts(c, AB_K_SPARSE_ZERO);
         launch_zero_sparse(p.bm, p.stride, p.words, sa.flag, nq, kSpAtomic, s);
     }
+    const uint32_t *kq = (const uint32_t *)c->kq.p;
+    if (p.Kcap) {
+        // the query of every key (kq), which the group masks use too, and the
+        // sparse queries' per-key record counts
+        TimedSpan ts(c, AB_K_SPARSE_PLAN);
+        launch_sparse_plan(d, p.keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
+    }
     {
         TimedSpan ts(c, AB_K_DENSE_MASK);
-        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
-                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskAny, s);
+        launch_mask_groups(p.mode, p.keys, kq, Kd, p.Kcap, (const uint32_t *)c->qgroup.p, c->gmask.p,
+                           p.mask_stride, (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskAny, s);
         launch_mask_rank((uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, p.ngroups, (const uint32_t *)c->plan.p, s);
-        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
-                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskSet, s);
+        launch_mask_groups(p.mode, p.keys, kq, Kd, p.Kcap, (const uint32_t *)c->qgroup.p, c->gmask.p,
+                           p.mask_stride, (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskSet, s);
     }
     if (p.Kcap) {
         {
             TimedSpan ts(c, AB_K_SPARSE_PLAN);
-            launch_sparse_plan(d, p.keys, qoff, sel, nq, (uint32_t *)c->kq.p, (uint64_t *)c->cnt.p, s);
             launch_scan_u64((const uint64_t *)c->cnt.p, (uint64_t *)c->koff.p, p.Kcap, (uint64_t *)c->parts.p, s,
                             Kd);
         }
@@ -179,8 +185,8 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     }
     {
         TimedSpan ts(c, AB_K_DENSE_MASK);
-        launch_mask_groups(p.mode, p.keys, qoff, (const uint32_t *)c->qgroup.p, nq, c->gmask.p, p.mask_stride,
-                           (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskClear, s);
+        launch_mask_groups(p.mode, p.keys, kq, Kd, p.Kcap, (const uint32_t *)c->qgroup.p, c->gmask.p,
+                           p.mask_stride, (uint32_t *)c->ganybm.p, p.any_stride, ix->nkeys, kMaskClear, s);
         launch_fill_zero(c->ganybm.p, (size_t)p.ngroups * p.any_stride * 4, s);
     }
     if (a.d_ids || a.d_counts) {

@@ -80,8 +80,10 @@ inline uint64_t mask_rank_words(uint32_t nkeys)
     const uint64_t nw = ((uint64_t)nkeys + 31) / 32;
     return (nw + 1) * 2 + ((nw + 63) / 64 + 3) / 4 * 4;
 }
-void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, const uint32_t *qgroup,
-                        uint32_t nq, void *masks, uint64_t mask_stride, uint32_t *anybms,
+// kq: the query of each key (k_sparse_plan); Kd: the key count on the device
+// (graph replays) or NULL for K.
+void launch_mask_groups(int mode, const uint32_t *keys, const uint32_t *kq, const uint64_t *Kd, uint64_t K,
+                        const uint32_t *qgroup, void *masks, uint64_t mask_stride, uint32_t *anybms,
                         uint64_t any_stride, uint32_t nkeys, int op, hipStream_t s);
 void launch_mask_rank(uint32_t *anybms, uint64_t any_stride, uint32_t nkeys, uint32_t ngroups, const uint32_t *plan,
                       hipStream_t s);

@@ -1023,29 +1023,29 @@ void launch_zero_sparse(uint32_t *bm, uint64_t stride, uint64_t words, const uin
 
 // op kMaskAny: set the keys' any-query bits; kMaskSet: OR each query's bit
 // into its keys' masks (after k_mask_rank); kMaskClear: zero those masks.
+// One thread per key of the batch (kq: the query of each key); the key count
+// comes from Kd on the device when it is set (graph replays).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict__ keys,
-                                                     const uint64_t *__restrict__ qoff,
-                                                     const uint32_t *__restrict__ qgroup, void *masks,
+                                                     const uint32_t *__restrict__ kq, const uint64_t *Kd,
+                                                     uint64_t K, const uint32_t *__restrict__ qgroup, void *masks,
                                                      uint64_t mask_stride, uint32_t *anybms, uint64_t any_stride,
                                                      uint32_t nkeys, int op)
 {
     using M = typename MaskOf<MODE>::T;
-    const uint32_t q = blockIdx.y;
-    const uint32_t qg = qgroup[q];
-    if (qg == kNoGroup) return;
-    const uint32_t g = qg / kGroupSlots, b = qg % kGroupSlots;
-    M *mask = reinterpret_cast<M *>(masks) + (uint64_t)g * mask_stride;
-    uint32_t *any = anybms + (uint64_t)g * any_stride;
-    const uint2 *rw = reinterpret_cast<const uint2 *>(any);
-    const uint64_t a = qoff[q], e = qoff[q + 1];
-    for (uint64_t j = a + (uint64_t)blockIdx.x * 256 + threadIdx.x; j < e; j += (uint64_t)gridDim.x * 256) {
+    const uint64_t n = Kd ? *Kd : K;
+    for (uint64_t j = (uint64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (uint64_t)gridDim.x * 256) {
         const uint32_t k = keys[j];
         if (k >= nkeys) continue;
+        const uint32_t qg = qgroup[kq[j]];
+        if (qg == kNoGroup) continue;
+        const uint32_t g = qg / kGroupSlots, b = qg % kGroupSlots;
+        uint32_t *any = anybms + (uint64_t)g * any_stride;
         if (op == kMaskAny) {
             atomicOr(&any[rank_word_index(k)], 1u << (k & 31));
         } else {
-            const uint32_t r = key_rank(rw[k >> 5], k);
+            M *mask = reinterpret_cast<M *>(masks) + (uint64_t)g * mask_stride;
+            const uint32_t r = key_rank(reinterpret_cast<const uint2 *>(any)[k >> 5], k);
             if (op == kMaskSet)
                 set_mask_bit(&mask[r], b);
             else
@@ -1054,29 +1054,24 @@ __global__ __launch_bounds__(256) void k_mask_groups(const uint32_t *__restrict_
     }
 }
 
-void launch_mask_groups(int mode, const uint32_t *keys, const uint64_t *qoff, const uint32_t *qgroup, uint32_t nq,
-                        void *masks, uint64_t mask_stride, uint32_t *anybms, uint64_t any_stride, uint32_t nkeys,
-                        int op, hipStream_t s)
+void launch_mask_groups(int mode, const uint32_t *keys, const uint32_t *kq, const uint64_t *Kd, uint64_t K,
+                        const uint32_t *qgroup, void *masks, uint64_t mask_stride, uint32_t *anybms,
+                        uint64_t any_stride, uint32_t nkeys, int op, hipStream_t s)
 {
-    for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
-        // a slice's kernels read qoff[q], qoff[q + 1] and qgroup[q] relative to its first query
-        const uint32_t m = nq - q0 < kMaxGridY ? nq - q0 : kMaxGridY;
-        const dim3 grid(8, m);
-        const uint64_t *qo = qoff + q0;
-        const uint32_t *qg = qgroup + q0;
-        if (mode == kModeMask32)
-            hipLaunchKernelGGL(k_mask_groups<kModeMask32>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, op);
-        else if (mode == kModeMask64)
-            hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, op);
-        else if (mode == kModeMask128)
-            hipLaunchKernelGGL(k_mask_groups<kModeMask128>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, op);
-        else
-            hipLaunchKernelGGL(k_mask_groups<kModeMask256>, grid, dim3(256), 0, s, keys, qo, qg, masks, mask_stride,
-                               anybms, any_stride, nkeys, op);
-    }
+    if (!K) return;
+    const dim3 grid((uint32_t)std::min<uint64_t>((K + 255) / 256, 8192));
+    if (mode == kModeMask32)
+        hipLaunchKernelGGL(k_mask_groups<kModeMask32>, grid, dim3(256), 0, s, keys, kq, Kd, K, qgroup, masks,
+                           mask_stride, anybms, any_stride, nkeys, op);
+    else if (mode == kModeMask64)
+        hipLaunchKernelGGL(k_mask_groups<kModeMask64>, grid, dim3(256), 0, s, keys, kq, Kd, K, qgroup, masks,
+                           mask_stride, anybms, any_stride, nkeys, op);
+    else if (mode == kModeMask128)
+        hipLaunchKernelGGL(k_mask_groups<kModeMask128>, grid, dim3(256), 0, s, keys, kq, Kd, K, qgroup, masks,
+                           mask_stride, anybms, any_stride, nkeys, op);
+    else
+        hipLaunchKernelGGL(k_mask_groups<kModeMask256>, grid, dim3(256), 0, s, keys, kq, Kd, K, qgroup, masks,
+                           mask_stride, anybms, any_stride, nkeys, op);
 }
 
 // Ranks of group g's first keys per 32, in two passes of one wave per 64 rank
