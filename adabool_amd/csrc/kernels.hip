@@ -1246,9 +1246,36 @@ __device__ __forceinline__ void sparse_emit_slice(const DevIndex &ix, const Fuse
     const uint32_t nch = a.sp_cpre[q + 1] - a.sp_cpre[q];
     const uint16_t *co = a.sp_choff + (uint64_t)a.sp_scpre[q] * (S + 1);
     const uint32_t *refs = a.sp_refs + a.sp_rbase[q];
-    for (uint32_t j = 0; j < nch; ++j, co += S + 1, refs += a.sp_chunk) {
-        const uint32_t lo = co[s], hi = co[s + 1];
-        for (uint32_t t = lo + threadIdx.x; t < hi; t += NT) emit_ref_lds(ix, refs[t], slice, (uint32_t)w0);
+    // up to NT route chunks at a time: every thread reads one chunk's range of
+    // this slice, a scan places the ranges, and the threads take the
+    // references of all of them together (one chunk after the other cost a
+    // dependent load pair per chunk)
+    __shared__ uint32_t s_lo[NT], s_pre[NT];
+    __shared__ unsigned long long s_sh[NT / 64];
+    for (uint32_t j0 = 0; j0 < nch; j0 += NT) {
+        const uint32_t j = j0 + threadIdx.x;
+        uint32_t lo = 0, n = 0;
+        if (j < nch) {
+            const uint16_t *c = co + (uint64_t)j * (S + 1);
+            lo = c[s];
+            n = c[s + 1] - lo;
+        }
+        unsigned long long tot;
+        const uint32_t ex = (uint32_t)block_excl_scan(n, s_sh, &tot);
+        s_lo[threadIdx.x] = lo;
+        s_pre[threadIdx.x] = ex;
+        __syncthreads();
+        const uint32_t m = nch - j0 < NT ? nch - j0 : NT;
+        for (uint32_t i = threadIdx.x; i < (uint32_t)tot; i += NT) {
+            uint32_t a0 = 0, a1 = m;  // the last chunk with s_pre <= i (it holds i)
+            while (a1 - a0 > 1) {
+                const uint32_t mid = (a0 + a1) >> 1;
+                if (s_pre[mid] <= i) a0 = mid; else a1 = mid;
+            }
+            const uint32_t t = s_lo[a0] + (i - s_pre[a0]);
+            emit_ref_lds(ix, refs[(uint64_t)(j0 + a0) * a.sp_chunk + t], slice, (uint32_t)w0);
+        }
+        __syncthreads();
     }
     __syncthreads();
     if (w0 >= a.out_words) return;  // block-uniform
