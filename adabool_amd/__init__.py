@@ -123,6 +123,7 @@ _SIGS = {
     "ab_query_batch_ids": (C.c_int, [_VP, _VP, _VP, C.c_uint32, _VP, _VP, _VP]),
     "ab_query_id_bound": (C.c_int, [_VP, _VP, C.c_uint32, _VP]),
     "ab_ctx_set_id_budget": (C.c_int, [_VP, C.c_uint64]),
+    "ab_ctx_set_id_method": (C.c_int, [_VP, C.c_int]),
     "ab_ctx_autotune": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, C.c_uint32, _VP, C.c_uint32, _VP]),
     "ab_server_config_default": (C.c_int, [_VP]),
     "ab_server_create": (C.c_int, [_VP, _VP, _VP]),
@@ -620,6 +621,10 @@ class Context:
 
     def set_id_budget(self, values: int):
         _check(_lib.ab_ctx_set_id_budget(self._h, int(values)))
+
+    def set_id_method(self, method: int):
+        """Id-only batches: 0 automatic, 1 always by sort, 2 by value slice wherever it applies."""
+        _check(_lib.ab_ctx_set_id_method(self._h, int(method)))
 
     def query_batch_ids_device(self, d_keys, qoff, d_ids, id_off, d_counts=None):
         """Id-only batch (no bitmaps): query i's ascending ids at

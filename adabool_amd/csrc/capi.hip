@@ -614,9 +614,9 @@ ab_ctx::~ab_ctx()
                       &mlp[0].params, &mlp[0].norm, &mlp[1].params, &mlp[1].norm, &qgroup, &grp_q, &grp_n, &plan,
                       &counter, &gmask, &ganybm, &idoff, &id_vals, &id_sorted, &id_ccnt, &id_cbase, &id_start,
                       &id_tmp, &gkeys, &sp_flag, &sp_cpre, &sp_scpre, &sp_rbase, &sp_slist, &sp_tot, &sp_refs,
-                      &sp_choff})
+                      &sp_choff, &idpart})
         b->release();
-    for (PinBuf *b : {&hstage, &hkstage, &hsel, &hM})
+    for (PinBuf *b : {&hstage, &hkstage, &hsel, &hM, &hidpart})
         b->release();
     if (stage_ev) (void)hipEventDestroy(stage_ev);
     if (kstage_ev) (void)hipEventDestroy(kstage_ev);

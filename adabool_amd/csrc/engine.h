@@ -94,7 +94,13 @@ struct ab_ctx {
     // most id_budget values, chunk counts and bases, sort output, run starts,
     // rocPRIM temporary storage
     uint64_t id_budget = 1ull << 27;
+    int id_method = 0;  // 0: by value slice where it applies, 1: always by sort
     ab::DevBuf id_vals, id_sorted, id_ccnt, id_cbase, id_start, id_tmp;
+    // id-only batches by value slice: per-part route arrays (device and their
+    // pinned staging), the work-groups of the slice kernel
+    ab::DevBuf idpart;
+    ab::PinBuf hidpart;
+    uint32_t ids_grid = 0;
     uint32_t gmask_groups = 0;  // groups the zeroed mask buffers hold
     int gmask_mode = -1;
     struct GraphCache {

@@ -17,6 +17,22 @@
 //   5. first copies flagged and ranked by a scan; the first index of every
 //      query's run; each first copy to the caller's offset for its query, and
 //      the number of first copies as its count.
+// A second method forms the ids with no sort, by value slice (for value
+// spaces of at most kSpMaxSlices slices of 2^18 values and record indices
+// that fit the 29-bit references):
+//   2'. per part: the records are numbered as above and routed by value
+//       slice into 4-byte references (the write-once sparse path's route,
+//       every query a slice query);
+//   3'. one persistent launch takes the (query, slice) items in order: the
+//       slice's references OR-ed into a 32 KB LDS bitmap, its bits counted,
+//       the query's ids before the slice found by a decoupled look-back over
+//       the query's earlier slices, and the bitmap expanded straight to ids.
+// Every item scans its whole slice, so the slices cost nq x S x 8192 words
+// whatever the result, and the sort costs follow the values: the automatic
+// choice takes the slices when the batch has at most kIdsSliceItems items or
+// its values fill at least 1/kIdsSliceDensity of the slices' value space
+// (measured on the C2 filter: 64-key queries are faster by slice up to batch
+// 256 and by sort from batch 1024; C3-mix queries faster by slice at any batch).
 // The result equals the bitmap paths' ids for every query: the set union of
 // its in-range keys' lists, ascending (test/adsfilter_q_all.cu:108-119,
 // 196-208).
@@ -56,6 +72,12 @@ constexpr uint32_t kIdsChunk = 4096;  // records per chunk
 constexpr uint32_t kIdsWin = 1024;    // key offsets per LDS window
 constexpr uint32_t kIdsGrid = 8192;   // work-groups of the chunk kernels (grid-stride)
 constexpr uint32_t kIdsStage = 16384; // LDS bytes staging one round's values
+// automatic method: by value slice up to this many (query, slice) items, or
+// when the values fill at least 1/kIdsSliceDensity of the slices
+constexpr uint64_t kIdsSliceItems = 8192;
+constexpr uint64_t kIdsSliceDensity = 20;
+// records per part of the slice method (2 GiB of references)
+constexpr uint64_t kIdsSliceRecords = 1ull << 29;
 
 // Exclusive work-group scan of one u32 per thread; *total = the sum.
 __device__ __forceinline__ uint32_t block_excl_u32(uint32_t x, uint32_t *sw, uint32_t *total)
@@ -404,6 +426,135 @@ extern "C" ab_status ab_ctx_set_id_budget(ab_ctx *c, uint64_t values)
     return AB_OK;
 }
 
+namespace {
+
+// Id-only parts by value slice (see the file comment): per part, route every
+// query's records into slice-sorted references, then one persistent launch of
+// the (query, slice) items.  tot: per query {records, values} from the plan.
+ab_status ids_by_slice(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq, uint32_t *d_ids,
+                       const uint64_t *h_id_off, uint32_t *counts, const std::vector<uint64_t> &tot,
+                       const std::vector<uint32_t> &part_q0, const std::vector<uint32_t> &part_q1, uint32_t S)
+{
+    ab_index *ix = c->ix;
+    const DevIndex &d = ix->dev;
+    hipStream_t s = c->stream;
+    const uint32_t chunk = c->sparse_chunk;
+    // ids + id_off[q] 16-byte aligned for every query: 128-bit stores
+    bool aligned = ((uintptr_t)d_ids % 16) == 0;
+    for (uint32_t q = 0; q < nq && aligned; ++q) aligned = h_id_off[q] % 4 == 0;
+    if (!c->sp_grid) {
+        int dev = 0, cus = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        c->sp_grid = (uint32_t)(cus * 4);
+    }
+    if (!c->ids_grid) c->ids_grid = ids_slices_grid();
+    // per-part route arrays, all parts staged at once (distinct pinned
+    // regions, so no copy of an earlier part is overwritten): qoff relative
+    // to the part's first key, refs bases, route-chunk prefix, totals, flags
+    auto a8 = [](size_t x) { return (x + 7) & ~(size_t)7; };
+    std::vector<size_t> at(part_q0.size() + 1, 0);
+    for (size_t pi = 0; pi < part_q0.size(); ++pi) {
+        const size_t n = part_q1[pi] - part_q0[pi];
+        at[pi + 1] = at[pi] + a8((n + 1) * 8) + a8(n * 8) + a8((n + 1) * 4) + 16 + a8(n);
+    }
+    AB_TRY(c->hidpart.ensure(at.back() + 64));
+    size_t dev_bytes = 0;
+    for (size_t pi = 0; pi < part_q0.size(); ++pi) dev_bytes = std::max(dev_bytes, at[pi + 1] - at[pi]);
+    AB_TRY(c->idpart.ensure(dev_bytes + 64));
+    for (size_t pi = 0; pi < part_q0.size(); ++pi) {
+        const uint32_t q0 = part_q0[pi], q1 = part_q1[pi], n = q1 - q0;
+        const uint64_t k0 = h_qoff[q0], Kp = h_qoff[q1] - k0;
+        uint8_t *h = (uint8_t *)c->hidpart.p + at[pi];
+        uint64_t *qoffp = (uint64_t *)h;
+        uint64_t *rbase = (uint64_t *)(h + a8((n + 1) * 8));
+        uint32_t *cpre = (uint32_t *)(h + a8((n + 1) * 8) + a8(n * 8));
+        uint32_t *ptot = cpre + ((a8((n + 1) * 4)) / 4);
+        uint8_t *flag = (uint8_t *)(ptot + 4);
+        uint64_t W = 0, nch = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t M = tot[2ull * (q0 + i)];
+            qoffp[i] = h_qoff[q0 + i] - k0;
+            rbase[i] = W;
+            cpre[i] = (uint32_t)nch;
+            flag[i] = kSpSlice;
+            W += M;
+            nch += (M + chunk - 1) / chunk;
+        }
+        qoffp[n] = Kp;
+        cpre[n] = (uint32_t)nch;
+        ptot[0] = (uint32_t)nch;
+        ptot[1] = n;
+        ptot[2] = (uint32_t)nch;
+        ptot[3] = 0;
+        if (nch >= (1ull << 32) || W >= (1ull << 32)) {
+            set_error("ab_query_batch_ids: part too large for the slice route");
+            return AB_UNSUPPORTED;
+        }
+        uint8_t *dp = (uint8_t *)c->idpart.p;
+        HIP_TRY(hipMemcpyAsync(dp, h, at[pi + 1] - at[pi], hipMemcpyHostToDevice, s));
+        // record numbering over the part's keys
+        AB_TRY(c->koff.ensure((Kp + 1) * 8));
+        AB_TRY(c->parts.ensure((scan_parts_needed(Kp) + 2) * 8));
+        launch_scan_u64((const uint64_t *)c->cnt.p + k0, (uint64_t *)c->koff.p, Kp, (uint64_t *)c->parts.p, s);
+        AB_TRY(c->sp_refs.ensure(W * 4 + 16));
+        AB_TRY(c->sp_choff.ensure(nch * (S + 1) * 2 + 16));
+        AB_TRY(c->id_ccnt.ensure((size_t)n * S * 8 + 64));
+        SparseArgs sa{};
+        sa.keys = d_keys + k0;
+        sa.kq = (const uint32_t *)c->kq.p + k0;
+        sa.qoff = (const uint64_t *)dp;
+        sa.koff = (const uint64_t *)c->koff.p;
+        sa.K = Kp;
+        sa.nq = n;
+        sa.chunk = chunk;
+        sa.nslices = S;
+        sa.flag = dp + ((uint8_t *)flag - h);
+        sa.cpre = (uint32_t *)(dp + ((uint8_t *)cpre - h));
+        sa.scpre = sa.cpre;
+        sa.rbase = (uint64_t *)(dp + ((uint8_t *)rbase - h));
+        sa.tot = (uint32_t *)(dp + ((uint8_t *)ptot - h));
+        sa.refs = (uint32_t *)c->sp_refs.p;
+        sa.choff = (uint16_t *)c->sp_choff.p;
+        sa.cap = W ? W : 1;
+        HIP_TRY(launch_sparse_route(d, sa, c->sp_grid, s));
+        // status words, work counter and error word start at zero
+        unsigned long long *st = (unsigned long long *)c->id_ccnt.p;
+        uint32_t *counter = (uint32_t *)(st + (size_t)n * S);
+        launch_fill_zero(st, (size_t)n * S * 8 + 16, s);
+        IdsSliceArgs ia{};
+        ia.cpre = sa.cpre;
+        ia.rbase = sa.rbase;
+        ia.refs = sa.refs;
+        ia.choff = sa.choff;
+        ia.chunk = chunk;
+        ia.nslices = S;
+        ia.nq = n;
+        ia.q0 = q0;
+        ia.id_off = (const uint64_t *)c->idoff.p;
+        ia.ids = d_ids;
+        ia.counts = counts;
+        ia.st = st;
+        ia.counter = counter;
+        ia.err = counter + 1;
+        HIP_TRY(launch_ids_slices(d, ia, aligned, c->ids_grid, s));
+        // the next part reuses koff, refs, choff and the status words
+    }
+    return AB_OK;
+}
+
+}  // namespace
+
+extern "C" ab_status ab_ctx_set_id_method(ab_ctx *c, int method)
+{
+    if (!c || method < 0 || method > 2) {
+        set_error("ab_ctx_set_id_method: 0 (automatic), 1 (by sort) or 2 (by value slice where it applies)");
+        return AB_INVALID_ARG;
+    }
+    c->id_method = method;
+    return AB_OK;
+}
+
 extern "C" ab_status ab_query_batch_ids(ab_ctx *c, const uint32_t *d_keys, const uint64_t *h_qoff, uint32_t nq,
                                         uint32_t *d_ids, const uint64_t *h_id_off, uint32_t *d_counts)
 {
@@ -455,22 +606,41 @@ extern "C" ab_status ab_query_batch_ids(ab_ctx *c, const uint32_t *d_keys, const
     }
     hipLaunchKernelGGL(k_ids_zero, dim3((nq + 255) / 256), dim3(256), 0, s, counts, nq);
 
-    // ---- parts of at most id_budget values (a larger query forms its own) ----
-    const uint64_t budget = c->id_budget;
+    // ---- the method ---------------------------------------------------------
+    const uint64_t S = std::max<uint64_t>(1, ((uint64_t)ix->nvalues + (1ull << kSpSliceShift) - 1) >> kSpSliceShift);
+    bool by_slice = S <= kSpMaxSlices && c->id_method != 1;
+    for (int t = 0; t < AB_NTYPES; ++t)
+        if (ix->nrec[t] >= (1ull << kSpTypeShift)) by_slice = false;
+    if (by_slice && c->id_method == 0) {
+        uint64_t T = 0;
+        for (uint32_t q = 0; q < nq; ++q) T += tot[2ull * q + 1];
+        const uint64_t items = (uint64_t)nq * S;
+        by_slice = items <= kIdsSliceItems || T * kIdsSliceDensity >= items << kSpSliceShift;
+    }
+
+    // ---- parts: by sort, of at most id_budget values (8 bytes each, twice);
+    // by slice, of at most 4 x id_budget records (4-byte references, the same
+    // scratch), up to kIdsSliceRecords; a larger query forms a part of its own
+    const uint64_t budget = by_slice ? std::min<uint64_t>(kIdsSliceRecords, 4 * c->id_budget) : c->id_budget;
+    const int what = by_slice ? 0 : 1;  // tot[2q]: records, tot[2q + 1]: values
     const int vbits = std::max(1, bits_for(ix->nvalues ? ix->nvalues - 1 : 0));
-    uint32_t q0 = 0;
-    while (q0 < nq) {
-        uint64_t T = tot[2ull * q0 + 1], W = tot[2ull * q0];
+    std::vector<uint32_t> part_q0, part_q1;
+    for (uint32_t q0 = 0; q0 < nq;) {
+        uint64_t T = tot[2ull * q0 + what], V = tot[2ull * q0 + 1];
         uint32_t q1 = q0 + 1;
-        while (q1 < nq && T + tot[2ull * q1 + 1] <= budget) {
-            T += tot[2ull * q1 + 1];
-            W += tot[2ull * q1];
-            ++q1;
+        while (q1 < nq && T + tot[2ull * q1 + what] <= budget) T += tot[2ull * q1 + what], V += tot[2ull * q1 + 1], ++q1;
+        if (V) {
+            part_q0.push_back(q0);
+            part_q1.push_back(q1);
         }
-        if (T == 0) {
-            q0 = q1;
-            continue;
-        }
+        q0 = q1;
+    }
+    if (by_slice) return ids_by_slice(c, d_keys, h_qoff, nq, d_ids, h_id_off, counts, tot, part_q0, part_q1,
+                                      (uint32_t)S);
+    for (size_t pi = 0; pi < part_q0.size(); ++pi) {
+        const uint32_t q0 = part_q0[pi], q1 = part_q1[pi];
+        uint64_t T = 0, W = 0;
+        for (uint32_t q = q0; q < q1; ++q) T += tot[2ull * q + 1], W += tot[2ull * q];
         const uint64_t k0 = h_qoff[q0], Kp = h_qoff[q1] - k0;
         const uint64_t nch = (W + kIdsChunk - 1) / kIdsChunk;
         // record numbering, chunk value counts, chunk bases
@@ -497,7 +667,6 @@ extern "C" ab_status ab_query_batch_ids(ab_ctx *c, const uint32_t *d_keys, const
         else
             AB_TRY(sort_and_write<unsigned long long>(c, d, keys_p, kq_p, Kp, W, grid, q0, vbits, vbits + qbits, T,
                                                       d_ids, counts));
-        q0 = q1;
     }
     return AB_OK;
 }

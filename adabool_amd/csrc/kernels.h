@@ -167,6 +167,25 @@ void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32
 void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
                       uint32_t *ccnt, uint32_t *ids, uint64_t ids_stride, const uint64_t *id_off,
                       bool off_aligned, uint32_t *counts, hipStream_t s);
+// Id-only batch by value slice (after launch_sparse_route has routed every
+// query's records into refs/choff as slice queries): item (qi, s) ORs slice
+// s of query qi in LDS, counts it and writes its ids to ids + id_off[q0 + qi]
+// after the query's earlier slices; counts[q0 + qi] = the total.  st: nq *
+// nslices status words and counter: zero at launch; err: set if a look-back
+// ever gave up.
+struct IdsSliceArgs {
+    const uint32_t *cpre;   // nq+1: route-chunk prefix over the part's queries
+    const uint64_t *rbase;  // nq: first ref of each query
+    const uint32_t *refs;
+    const uint16_t *choff;  // per route chunk: nslices+1 offsets into its refs
+    uint32_t chunk, nslices, nq, q0;
+    const uint64_t *id_off;
+    uint32_t *ids, *counts;
+    unsigned long long *st;
+    uint32_t *counter, *err;
+};
+uint32_t ids_slices_grid();
+hipError_t launch_ids_slices(const DevIndex &ix, const IdsSliceArgs &a, bool aligned, uint32_t grid, hipStream_t s);
 void launch_features(const DevIndex &ix, const uint32_t *keys, const uint64_t *qoff, uint32_t nq,
                      uint64_t R, float *feats, uint64_t *M, hipStream_t s);
 void launch_decide_threshold(const uint64_t *M, uint32_t nq, double thr, uint8_t *sel, hipStream_t s);

@@ -1233,25 +1233,18 @@ __device__ __forceinline__ void emit_ref_lds(const DevIndex &ix, uint32_t ref, u
     }
 }
 
-// Slice s of slice query q: its records from every route chunk of q, OR-ed
-// into LDS, then every word of the slice written once.
+// ORs slice s of one slice query's routed records (nch route chunks of
+// `chunk` references each, co: their per-chunk slice offsets) into the LDS
+// slice whose first word is w0.  Up to NT chunks at a time: every thread reads
+// one chunk's range of this slice, a scan places the ranges, and the threads
+// take the references of all of them together (one chunk after the other cost
+// a dependent load pair per chunk).  Ends with a barrier.
 template <int NT>
-__device__ __forceinline__ void sparse_emit_slice(const DevIndex &ix, const FusedArgs &a, uint32_t q, uint32_t s,
-                                                  uint32_t *slice)
+__device__ __forceinline__ void or_slice_refs(const DevIndex &ix, const uint16_t *co, const uint32_t *refs,
+                                              uint32_t nch, uint32_t S, uint32_t chunk, uint32_t s, uint32_t w0,
+                                              uint32_t *slice, uint32_t *s_lo, uint32_t *s_pre,
+                                              unsigned long long *s_sh)
 {
-    const uint64_t w0 = (uint64_t)s * kSpSliceWords;
-    for (uint32_t i = threadIdx.x; i < kSpSliceWords; i += NT) slice[i] = 0;
-    __syncthreads();
-    const uint32_t S = a.nslices;
-    const uint32_t nch = a.sp_cpre[q + 1] - a.sp_cpre[q];
-    const uint16_t *co = a.sp_choff + (uint64_t)a.sp_scpre[q] * (S + 1);
-    const uint32_t *refs = a.sp_refs + a.sp_rbase[q];
-    // up to NT route chunks at a time: every thread reads one chunk's range of
-    // this slice, a scan places the ranges, and the threads take the
-    // references of all of them together (one chunk after the other cost a
-    // dependent load pair per chunk)
-    __shared__ uint32_t s_lo[NT], s_pre[NT];
-    __shared__ unsigned long long s_sh[NT / 64];
     for (uint32_t j0 = 0; j0 < nch; j0 += NT) {
         const uint32_t j = j0 + threadIdx.x;
         uint32_t lo = 0, n = 0;
@@ -1273,11 +1266,28 @@ __device__ __forceinline__ void sparse_emit_slice(const DevIndex &ix, const Fuse
                 if (s_pre[mid] <= i) a0 = mid; else a1 = mid;
             }
             const uint32_t t = s_lo[a0] + (i - s_pre[a0]);
-            emit_ref_lds(ix, refs[(uint64_t)(j0 + a0) * a.sp_chunk + t], slice, (uint32_t)w0);
+            emit_ref_lds(ix, refs[(uint64_t)(j0 + a0) * chunk + t], slice, w0);
         }
         __syncthreads();
     }
+}
+
+// Slice s of slice query q: its records from every route chunk of q, OR-ed
+// into LDS, then every word of the slice written once.
+template <int NT>
+__device__ __forceinline__ void sparse_emit_slice(const DevIndex &ix, const FusedArgs &a, uint32_t q, uint32_t s,
+                                                  uint32_t *slice)
+{
+    const uint64_t w0 = (uint64_t)s * kSpSliceWords;
+    for (uint32_t i = threadIdx.x; i < kSpSliceWords; i += NT) slice[i] = 0;
     __syncthreads();
+    const uint32_t S = a.nslices;
+    const uint32_t nch = a.sp_cpre[q + 1] - a.sp_cpre[q];
+    const uint16_t *co = a.sp_choff + (uint64_t)a.sp_scpre[q] * (S + 1);
+    const uint32_t *refs = a.sp_refs + a.sp_rbase[q];
+    __shared__ uint32_t s_lo[NT], s_pre[NT];
+    __shared__ unsigned long long s_sh[NT / 64];
+    or_slice_refs<NT>(ix, co, refs, nch, S, a.sp_chunk, s, (uint32_t)w0, slice, s_lo, s_pre, s_sh);
     if (w0 >= a.out_words) return;  // block-uniform
     const uint64_t nw = a.out_words - w0 < kSpSliceWords ? a.out_words - w0 : kSpSliceWords;
     uint32_t *dst = a.out + (uint64_t)q * a.out_stride + w0;
@@ -1566,6 +1576,164 @@ void launch_write_ids(const uint32_t *bm, uint64_t stride, uint64_t words, uint3
             hipLaunchKernelGGL(k_write_ids<false>, dim3(nchunk, n), dim3(256), 0, s, b, stride, words, nchunk, cc,
                                dst, ids_stride, off);
     }
+}
+
+// ---------------------------------------------------------------------------
+// id-only batches by value slice: each query's routed records OR-ed into an
+// LDS slice bitmap, counted, and expanded straight to ids (no bitmap in HBM,
+// no sort)
+// ---------------------------------------------------------------------------
+// Status word of a (query, slice) item in the one-pass prefix: kLbInc |
+// inclusive prefix, kLbAgg | the item's own count, or 0 (not yet published);
+// read and written with relaxed agent-scope atomics (each carries its own
+// value, so no other memory is ordered by it).
+constexpr unsigned long long kLbAgg = 1ull << 62, kLbInc = 1ull << 63, kLbVal = kLbAgg - 1;
+// A look-back that polls this often gives up and sets the error word (a
+// guard against a hang; a correct launch never comes near it).
+constexpr uint32_t kLbSpinMax = 1u << 22;
+
+__device__ __forceinline__ unsigned long long lb_load(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of item j0 (> 0) among the items whose status words start
+// at sq, by one wave: lane l reads item j - 1 - l, and each window of 64 is
+// summed up to its first inclusive prefix.  Items are taken in ticket order,
+// so every item waited on belongs to a running work-group that publishes its
+// count before it waits itself.
+__device__ __forceinline__ unsigned long long lb_prefix(unsigned long long *sq, uint32_t j0, uint32_t *err)
+{
+    const uint32_t lane = lane_id();
+    unsigned long long excl = 0;
+    uint32_t j = j0, spins = 0;
+    while (j > 0) {
+        const bool valid = lane < j;
+        const uint32_t idx = j - 1 - lane;
+        // lanes before item 0 never count: item 0 always holds an inclusive prefix
+        unsigned long long v = valid ? lb_load(&sq[idx]) : kLbInc;
+        for (;;) {
+            const unsigned long long incm = __ballot((v & kLbInc) != 0);
+            const uint32_t first = incm ? (uint32_t)__builtin_ctzll(incm) : 64u;
+            const unsigned long long need = first >= 63 ? ~0ull : ((2ull << first) - 1);
+            if (!(__ballot(v == 0) & need)) {
+                excl += wave_sum64(lane <= first ? (v & kLbVal) : 0ull);
+                j = first < 64 ? 0 : j - 64;
+                break;
+            }
+            if (++spins > kLbSpinMax) {
+                if (lane == 0) atomicOr(err, 1u);
+                return excl;
+            }
+            if (v == 0) v = lb_load(&sq[idx]);
+        }
+    }
+    return excl;
+}
+
+// Persistent: items (query qi, slice s) in ticket order.  ORs the slice's
+// routed records into LDS, counts them in steps of 256 words (wave w takes
+// words 256 i + 64 w ..), publishes the count, finds the query's ids before
+// the slice by look-back, then expands each 64-word piece to ids.  The last
+// slice of a query writes its count.
+template <bool ALIGNED>
+__global__ __launch_bounds__(256) void k_ids_slices(DevIndex ix, IdsSliceArgs a)
+{
+    constexpr uint32_t NT = 256, NS = kSpSliceWords / 256;
+    __shared__ __attribute__((aligned(16))) uint32_t slice[kSpSliceWords];
+    __shared__ __attribute__((aligned(16))) uint32_t stage[4][kStage];
+    __shared__ uint32_t wcnt[NS][4];
+    __shared__ uint32_t s_item, s_any;
+    __shared__ unsigned long long s_excl;
+    __shared__ uint32_t s_lo[NT], s_pre[NT];
+    __shared__ unsigned long long s_sh[NT / 64];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    const uint32_t S = a.nslices;
+    const uint64_t nitems = (uint64_t)a.nq * S;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            s_item = atomicAdd(a.counter, 1u);
+            s_any = 0;
+        }
+        __syncthreads();
+        const uint32_t item = s_item;
+        if (item >= nitems) break;
+        const uint32_t qi = item / S, s = item - qi * S;
+        const uint32_t w0 = s * kSpSliceWords;
+        const uint32_t nch = a.cpre[qi + 1] - a.cpre[qi];
+        const uint16_t *co = a.choff + (uint64_t)a.cpre[qi] * (S + 1);
+        const uint32_t *refs = a.refs + a.rbase[qi];
+        // does any chunk hold records of this slice?
+        for (uint32_t j = threadIdx.x; j < nch; j += NT)
+            if (co[(uint64_t)j * (S + 1) + s + 1] != co[(uint64_t)j * (S + 1) + s]) s_any = 1;
+        for (uint32_t i = threadIdx.x; i < kSpSliceWords / 4; i += NT)
+            reinterpret_cast<uint4 *>(slice)[i] = make_uint4(0, 0, 0, 0);
+        __syncthreads();
+        const bool any = s_any != 0;
+        if (any) {
+            or_slice_refs<NT>(ix, co, refs, nch, S, a.chunk, s, w0, slice, s_lo, s_pre, s_sh);
+#pragma unroll 1
+            for (uint32_t i = 0; i < NS; ++i) {
+                const uint32_t n = (uint32_t)__builtin_amdgcn_readlane(
+                    (int)wave_incl_scan_dpp((uint32_t)__popc(slice[i * 256 + wave * 64 + lane])), 63);
+                if (lane == 0) wcnt[i][wave] = n;
+            }
+        } else if (threadIdx.x < 4 * NS) {
+            wcnt[threadIdx.x >> 2][threadIdx.x & 3] = 0;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t mine = 0;
+            for (uint32_t i = lane; i < 4 * NS; i += 64) mine += wcnt[i >> 2][i & 3];
+            const uint32_t agg = wave_sum(mine);
+            unsigned long long *sq = a.st + (uint64_t)qi * S;
+            if (lane == 0) lb_store(&sq[s], (s == 0 ? kLbInc : kLbAgg) | agg);
+            const unsigned long long excl = s ? lb_prefix(sq, s, a.err) : 0ull;
+            if (lane == 0) {
+                if (s) lb_store(&sq[s], kLbInc | (excl + agg));
+                if (s + 1 == S) a.counts[a.q0 + qi] = (uint32_t)(excl + agg);
+                s_excl = excl;
+            }
+        }
+        __syncthreads();
+        if (any) {
+            uint64_t run = s_excl;
+            uint32_t *dst = a.ids + a.id_off[a.q0 + qi];
+            uint32_t *stg = stage[wave];
+#pragma unroll 1
+            for (uint32_t i = 0; i < NS; ++i) {
+                uint64_t off = run;
+                for (uint32_t w = 0; w < wave; ++w) off += wcnt[i][w];
+                run += wcnt[i][0] + wcnt[i][1] + wcnt[i][2] + wcnt[i][3];
+                const uint32_t word = i * 256 + wave * 64 + lane;
+                expand_piece<ALIGNED>(slice[word], (w0 + word) * 32, off, dst, stg, lane);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+uint32_t ids_slices_grid()
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev))
+        return 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_ids_slices<true>, 256, 0) != hipSuccess || per < 1)
+        per = 1;
+    return (uint32_t)(cus * per);
+}
+
+hipError_t launch_ids_slices(const DevIndex &ix, const IdsSliceArgs &a, bool aligned, uint32_t grid, hipStream_t s)
+{
+    if (aligned)
+        hipLaunchKernelGGL(k_ids_slices<true>, dim3(grid), dim3(256), 0, s, ix, a);
+    else
+        hipLaunchKernelGGL(k_ids_slices<false>, dim3(grid), dim3(256), 0, s, ix, a);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

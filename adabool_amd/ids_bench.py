@@ -64,8 +64,11 @@ def run_batches(ctx, d_keys, qo, ub, b, nrun, fn):
     def sub(s):
         return d_keys[int(qo[s]):] if int(qo[s]) < nk else d_keys, (qo[s:s + b + 1] - qo[s]).astype(np.uint64)
 
-    k0, q0 = sub(0)
-    fn(k0, q0, d_ids, offs[0], d_cnt)  # warm-up (allocations, graph capture)
+    # warm-up over every batch once: the context's scratch grows to the
+    # largest batch (and graphs are captured) before the timed pass
+    for j, s in enumerate(range(0, nrun, b)):
+        ks, qs = sub(s)
+        fn(ks, qs, d_ids, offs[j], d_cnt)
     ctx.sync()
     lat = []
     t0 = time.perf_counter()
@@ -120,6 +123,9 @@ def main(argv=None):
     ap.add_argument("--wide-baseline-queries", type=int, default=256)
     ap.add_argument("--model", default=os.path.join(ROOT, "adabool_amd", "models", "cost_model_c4.json"))
     ap.add_argument("--tuning", default=os.path.join(ROOT, "adabool_amd", "models", "tuning_c2.json"))
+    ap.add_argument("--id-method", type=int, default=0, choices=[0, 1, 2],
+                    help="id-only method: 0 automatic, 1 by sort, 2 by value slice where it applies")
+    ap.add_argument("--no-wide", action="store_true", help="skip the nvalues = 2^32 - 1 filter")
     a = ap.parse_args(argv)
 
     import torch
@@ -144,6 +150,7 @@ def main(argv=None):
         ts = next(iter(next(iter(tuning["entries"].values())).values())).get("tile_shift", 0)
     ix = Index(pk, pv, spec.nkeys, spec.nvalues, ts)
     ctx = Context(ix)
+    ctx.set_id_method(a.id_method)
     if os.path.exists(a.model):
         d = ctx.load_model(a.model)
         ctx.set_threshold(d["single"]["meta"]["tau"])
@@ -156,53 +163,55 @@ def main(argv=None):
     ix.close()
 
     # ---- C2 wide: values over the whole u32 range -----------------------------
-    wv = (pv.astype(np.uint64) * WIDE_MUL % (1 << 32)).astype(np.uint32)
-    keep = wv != 0xFFFFFFFF  # the one value that nvalues = 2^32 - 1 cannot hold
-    wk, wv = pk[keep], wv[keep]
-    nv_wide = 2**32 - 1
-    t = time.time()
-    ixw = Index.build_device(wk, wv, spec.nkeys, nv_wide)
-    log(f"wide index (device build) in {time.time() - t:.1f}s: {ixw.info['nrec_total']} records, "
-        f"bitmap {ixw.stride_bytes / 2**20:.0f} MiB per query")
-    ubw = id_bounds(wk, spec.nkeys, nv_wide, qk, qo)
-    ctx = Context(ixw)
-    wide_ref = None
-    runs = []
-    for b in batches:
-        nrun = min(nq, max(a.min_queries, b * 4))
-        nrun -= nrun % b
-        iq, il, ic = run_batches(ctx, d_keys, qo, ubw, b, nrun,
-                                 lambda k, q, i, o, c: ctx.query_batch_ids_device(k, q, i, o, c))
-        if wide_ref is None:
-            wide_ref = ic
-        ok = bool(np.array_equal(ic[: wide_ref.size], wide_ref[: ic.size]))
-        runs.append({"batch": b, "queries": nrun, "ids_only_qps": round(iq, 1), "batch_p99_ms": stats(il)["p99"],
-                     "counts_equal_batch1": ok})
-        log(f"wide batch {b}: id-only {iq:.0f} q/s, counts equal: {ok}")
-    # the bitmap path fits one query at a time: 512 MB bitmap, ids at the bound
-    nb = min(a.wide_baseline_queries, nq)
-    d_bm = torch.empty(ixw.stride_words, dtype=torch.int32, device="cuda")
-    d_ids = torch.empty(int(max(ubw[:nb].max(), 4)), dtype=torch.int32, device="cuda")
-    d_c = torch.empty(1, dtype=torch.int32, device="cuda")
-    ctx.query_device(qk[qo[0]:qo[1]], AB_PATH_THRESHOLD, d_bm, d_ids, d_c)
-    ctx.sync()
-    lat, cnt = [], np.zeros(nb, np.int64)
-    t0 = time.perf_counter()
-    for i in range(nb):
-        s0 = time.perf_counter()
-        ctx.query_device(qk[qo[i]:qo[i + 1]], AB_PATH_THRESHOLD, d_bm, d_ids, d_c)
+    if not a.no_wide:
+        wv = (pv.astype(np.uint64) * WIDE_MUL % (1 << 32)).astype(np.uint32)
+        keep = wv != 0xFFFFFFFF  # the one value that nvalues = 2^32 - 1 cannot hold
+        wk, wv = pk[keep], wv[keep]
+        nv_wide = 2**32 - 1
+        t = time.time()
+        ixw = Index.build_device(wk, wv, spec.nkeys, nv_wide)
+        log(f"wide index (device build) in {time.time() - t:.1f}s: {ixw.info['nrec_total']} records, "
+            f"bitmap {ixw.stride_bytes / 2**20:.0f} MiB per query")
+        ubw = id_bounds(wk, spec.nkeys, nv_wide, qk, qo)
+        ctx = Context(ixw)
+        ctx.set_id_method(a.id_method)
+        wide_ref = None
+        runs = []
+        for b in batches:
+            nrun = min(nq, max(a.min_queries, b * 4))
+            nrun -= nrun % b
+            iq, il, ic = run_batches(ctx, d_keys, qo, ubw, b, nrun,
+                                     lambda k, q, i, o, c: ctx.query_batch_ids_device(k, q, i, o, c))
+            if wide_ref is None:
+                wide_ref = ic
+            ok = bool(np.array_equal(ic[: wide_ref.size], wide_ref[: ic.size]))
+            runs.append({"batch": b, "queries": nrun, "ids_only_qps": round(iq, 1), "batch_p99_ms": stats(il)["p99"],
+                         "counts_equal_batch1": ok})
+            log(f"wide batch {b}: id-only {iq:.0f} q/s, counts equal: {ok}")
+        # the bitmap path fits one query at a time: 512 MB bitmap, ids at the bound
+        nb = min(a.wide_baseline_queries, nq)
+        d_bm = torch.empty(ixw.stride_words, dtype=torch.int32, device="cuda")
+        d_ids = torch.empty(int(max(ubw[:nb].max(), 4)), dtype=torch.int32, device="cuda")
+        d_c = torch.empty(1, dtype=torch.int32, device="cuda")
+        ctx.query_device(qk[qo[0]:qo[1]], AB_PATH_THRESHOLD, d_bm, d_ids, d_c)
         ctx.sync()
-        lat.append((time.perf_counter() - s0) * 1e3)
-        cnt[i] = int(d_c.cpu().numpy().view(np.uint32)[0])
-    base = nb / (time.perf_counter() - t0)
-    st = stats(lat)
-    same = bool(np.array_equal(cnt, wide_ref[:nb])) if wide_ref.size >= nb else None
-    out["wide"] = {"nvalues": nv_wide, "pairs": int(wk.size), "bitmap_bytes_per_query": ixw.stride_bytes,
-                   "runs": runs, "bitmap_per_query": {"qps": round(base, 1), "p50_ms": st["p50"], "p99_ms": st["p99"],
-                                                      "queries": nb, "counts_equal_ids_only": same},
-                   "best_ids_only_vs_bitmap_per_query": round(max(r["ids_only_qps"] for r in runs) / base, 1)}
-    log(f"wide bitmap path per query: {base:.0f} q/s (p50 {st['p50']:.3f} ms), counts equal: {same}")
-    ctx.close()
+        lat, cnt = [], np.zeros(nb, np.int64)
+        t0 = time.perf_counter()
+        for i in range(nb):
+            s0 = time.perf_counter()
+            ctx.query_device(qk[qo[i]:qo[i + 1]], AB_PATH_THRESHOLD, d_bm, d_ids, d_c)
+            ctx.sync()
+            lat.append((time.perf_counter() - s0) * 1e3)
+            cnt[i] = int(d_c.cpu().numpy().view(np.uint32)[0])
+        base = nb / (time.perf_counter() - t0)
+        st = stats(lat)
+        same = bool(np.array_equal(cnt, wide_ref[:nb])) if wide_ref.size >= nb else None
+        out["wide"] = {"nvalues": nv_wide, "pairs": int(wk.size), "bitmap_bytes_per_query": ixw.stride_bytes,
+                       "runs": runs, "bitmap_per_query": {"qps": round(base, 1), "p50_ms": st["p50"], "p99_ms": st["p99"],
+                                                          "queries": nb, "counts_equal_ids_only": same},
+                       "best_ids_only_vs_bitmap_per_query": round(max(r["ids_only_qps"] for r in runs) / base, 1)}
+        log(f"wide bitmap path per query: {base:.0f} q/s (p50 {st['p50']:.3f} ms), counts equal: {same}")
+        ctx.close()
     print(json.dumps(out), flush=True)
 
 

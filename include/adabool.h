@@ -321,12 +321,21 @@ ab_status ab_query_batch_packed(ab_ctx *ctx, const uint32_t *d_keys, const uint6
 
 /* Id-only batch: query i's ascending ids at d_ids + h_id_off[i] (u32 units)
  * and its count at d_counts[i] (NULL: not returned), formed without any
- * result bitmap.  The query's list records are expanded into values, then
- * radix-sorted and deduplicated on the device, so memory and traffic follow
- * the values its lists hold, not nvalues: it serves indices whose per-query
- * bitmaps would not fit (nvalues up to 2^32 - 1) and queries whose results
- * are a small part of nvalues.  Query i needs room for ab_query_id_bound ids;
- * ranges must not overlap.  The batch runs in parts of at most the context's
+ * result bitmap of nvalues bits in device memory.  Two methods:
+ *   by sort: the records are expanded into values, radix-sorted and
+ *   deduplicated on the device, so memory and traffic follow the values the
+ *   lists hold, whatever nvalues (up to 2^32 - 1);
+ *   by value slice (nvalues <= 2^28 and every record type below 2^29
+ *   records): the records are routed by slice of 2^18 values and each
+ *   (query, slice) is OR-ed into a 32 KB LDS bitmap and expanded straight to
+ *   ids; its cost follows nq x slices, so it serves small batches and dense
+ *   results.
+ * The automatic choice (ab_ctx_set_id_method) takes the slices for batches
+ * of at most 8192 (query, slice) items or whose values fill at least 1/20 of
+ * the slices' value space, else the sort.  It serves indices whose per-query
+ * bitmaps would not fit and queries whose results are a small part of
+ * nvalues.  Query i needs room for ab_query_id_bound ids; ranges must not
+ * overlap.  The batch runs in parts of at most the context's
  * id budget of expanded values (ab_ctx_set_id_budget; a larger query forms a
  * part of its own), with one host round trip per call.  The result equals
  * ab_query_batch's ids for every query.  No reference counterpart: the
@@ -339,9 +348,13 @@ ab_status ab_query_batch_ids(ab_ctx *ctx, const uint32_t *d_keys, const uint64_t
  * keys' list lengths (repeats included), capped at nvalues. */
 ab_status ab_query_id_bound(const ab_index *ix, const uint32_t *h_keys, uint32_t n,
                             uint64_t *bound);
-/* Expanded values per part of ab_query_batch_ids (>= 4096; default 2^27,
- * 3 GiB of scratch). */
+/* Expanded values per part of ab_query_batch_ids by sort (>= 4096; default
+ * 2^27, 3 GiB of scratch); the slice method takes parts of at most 4x as many
+ * records (4-byte references), and at most 2^29. */
 ab_status ab_ctx_set_id_budget(ab_ctx *ctx, uint64_t values);
+/* Method of ab_query_batch_ids: 0 (default) automatic; 1 always by sort;
+ * 2 by value slice wherever it applies (else by sort). */
+ab_status ab_ctx_set_id_method(ab_ctx *ctx, int method);
 
 /* Bitmap -> ascending ids for nq bitmaps at d_bitmaps + i * stride
  * (CollectIdsFromBitmap, test/adsfilter_q_all.cu:108-119, on the device). */

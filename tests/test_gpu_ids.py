@@ -10,8 +10,10 @@
   512 MB per query, on host- and device-built indices;
 * the serving loop with ids_only: alone == in mix == oracle, and replay
   counts equal the oracle's;
-* a filter of 1.06M values with a heavy key over the whole value range, at
-  aligned and unaligned offsets, in one part and in many."""
+* both methods (by sort, by value slice) and the automatic choice; a filter
+  of 1.06M values (five value slices, the last partial) with a heavy key over
+  the whole value range, at aligned and unaligned offsets, in one part and in
+  many."""
 import numpy as np
 import pytest
 
@@ -79,12 +81,14 @@ def _check_ranges(h, off, pos, counts, what):
         assert (h[off[i] + int(counts[i]):end] == 0xFFFFFFFF).all(), f"{what} q{i} wrote past its range"
 
 
+@pytest.mark.parametrize("method", [0, 1, 2])
 @pytest.mark.parametrize("aligned", [True, False])
-def test_ids_c1_mix(c1, aligned):
+def test_ids_c1_mix(c1, aligned, method):
     ix, csr = c1
     ctx = ab.Context(ix)
+    ctx.set_id_method(method)
     qs = _mix(0xADAB0601, 60)
-    print(f"ids c1 mix aligned={aligned}", flush=True)
+    print(f"ids c1 mix aligned={aligned} method={method}", flush=True)
     got, counts, h, off, pos = _run_ids(ctx, qs, aligned)
     for i, q in enumerate(qs):
         want = csr.union_ids(q)
@@ -158,6 +162,8 @@ def test_ids_budget_parts(c1, budget):
     _check_ranges(h, off, pos, counts, f"budget={budget}")
     with pytest.raises(ab.AdaBoolError):
         ctx.set_id_budget(100)
+    with pytest.raises(ab.AdaBoolError):
+        ctx.set_id_method(3)
     ctx.close()
 
 
@@ -250,14 +256,18 @@ def wide():
     return ix, csr, qs
 
 
+@pytest.mark.parametrize("method", [1, 2])
 @pytest.mark.parametrize("budget", [0, 60000])
 @pytest.mark.parametrize("aligned", [True, False])
-def test_ids_wide_filter(wide, aligned, budget):
+def test_ids_wide_filter(wide, aligned, budget, method):
+    """Both methods (by sort, by value slice) on five value slices, the last
+    partial, in one part and in many."""
     ix, csr, qs = wide
     ctx = ab.Context(ix)
+    ctx.set_id_method(method)
     if budget:
         ctx.set_id_budget(budget)
-    print(f"ids wide filter aligned={aligned} budget={budget}", flush=True)
+    print(f"ids wide filter aligned={aligned} budget={budget} method={method}", flush=True)
     ids, counts, h, off, pos = _run_ids(ctx, qs, aligned)
     _check_ranges(h, off, pos, counts, f"aligned={aligned} budget={budget}")
     for i, q in enumerate(qs):
