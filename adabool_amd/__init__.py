@@ -124,6 +124,8 @@ _SIGS = {
     "ab_query_id_bound": (C.c_int, [_VP, _VP, C.c_uint32, _VP]),
     "ab_ctx_set_id_budget": (C.c_int, [_VP, C.c_uint64]),
     "ab_ctx_set_id_method": (C.c_int, [_VP, C.c_int]),
+    "ab_ctx_set_batch_tuning": (C.c_int, [_VP, _VP, C.c_uint32]),
+    "ab_ctx_get_batch_settings": (C.c_int, [_VP, _VP]),
     "ab_ctx_autotune": (C.c_int, [_VP, _VP, _VP, C.c_uint32, C.c_int, C.c_uint32, _VP, C.c_uint32, _VP]),
     "ab_server_config_default": (C.c_int, [_VP]),
     "ab_server_create": (C.c_int, [_VP, _VP, _VP]),
@@ -609,6 +611,23 @@ class Context:
         self.set_sparse_chunk(int(cfg["sparse_chunk"]))
         self.set_dense_columns(bool(cfg["dense_columns"]))
 
+    def set_batch_tuning(self, table):
+        """Launch settings by batch size: a list of dicts with min_batch and
+        the apply_tuning keys, in ascending min_batch; each batch applies the
+        entry with the largest min_batch <= its query count ([] removes it)."""
+        arr = (_Tuning * max(len(table), 1))()
+        for i, e in enumerate(table):
+            arr[i] = _Tuning(int(e["min_batch"]), int(e["group"]), int(e["dense_threads"]), int(e["sparse_chunk"]),
+                             1 if e["dense_columns"] else 0)
+        _check(_lib.ab_ctx_set_batch_tuning(self._h, C.byref(arr), len(table)))
+
+    def batch_settings(self) -> dict:
+        """The launch settings in force (after the last batch's table entry)."""
+        t = _Tuning()
+        _check(_lib.ab_ctx_get_batch_settings(self._h, C.byref(t)))
+        return {"group": t.group, "dense_threads": t.dense_threads, "sparse_chunk": t.sparse_chunk,
+                "dense_columns": bool(t.dense_columns)}
+
     def query_batch_packed(self, d_keys, qoff, path: int, d_ids, id_off, d_bitmaps=None, d_counts=None) -> np.ndarray:
         """Batch with query i's ids written at d_ids + id_off[i] (u32 units)."""
         off = np.ascontiguousarray(np.asarray(qoff, dtype=np.uint64))
@@ -754,6 +773,11 @@ class Context:
 # ---------------------------------------------------------------------------
 # the reference's own interface (ivf_list and its workers)
 # ---------------------------------------------------------------------------
+class _Tuning(C.Structure):
+    _fields_ = [("min_batch", C.c_uint32), ("group", C.c_uint32), ("dense_threads", C.c_uint32),
+                ("sparse_chunk", C.c_uint32), ("dense_columns", C.c_int32)]
+
+
 class _TuneConfig(C.Structure):
     _fields_ = [("group", C.c_uint32), ("dense_threads", C.c_uint32), ("sparse_chunk", C.c_uint32),
                 ("dense_columns", C.c_int), ("ms", C.c_double), ("mismatched_words", C.c_uint64)]

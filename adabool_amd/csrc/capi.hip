@@ -707,6 +707,58 @@ extern "C" ab_status ab_ctx_set_dense_columns(ab_ctx *c, int on)
     return AB_OK;
 }
 
+extern "C" ab_status ab_ctx_set_batch_tuning(ab_ctx *c, const ab_tuning *e, uint32_t n)
+{
+    if (!c || (n && !e)) {
+        set_error("ab_ctx_set_batch_tuning: bad argument");
+        return AB_INVALID_ARG;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const ab_tuning &t = e[i];
+        const bool ok = t.min_batch >= 1 && (i == 0 || t.min_batch > e[i - 1].min_batch) && t.group >= 1 &&
+                        t.group <= kGroupSlots &&
+                        (t.dense_threads == 64 || t.dense_threads == 128 || t.dense_threads == 256 ||
+                         t.dense_threads == 512) &&
+                        t.sparse_chunk >= 256 && t.sparse_chunk <= 8192 && !(t.sparse_chunk & (t.sparse_chunk - 1));
+        if (!ok) {
+            set_error("ab_ctx_set_batch_tuning: entry " + std::to_string(i) +
+                      " out of range or min_batch not strictly ascending");
+            return AB_INVALID_ARG;
+        }
+    }
+    c->tuning.assign(e, e + n);
+    return AB_OK;
+}
+
+extern "C" ab_status ab_ctx_get_batch_settings(const ab_ctx *c, ab_tuning *out)
+{
+    if (!c || !out) {
+        set_error("ab_ctx_get_batch_settings: NULL argument");
+        return AB_INVALID_ARG;
+    }
+    *out = ab_tuning{0, c->group, c->dense_threads, c->sparse_chunk, c->dense_columns ? 1 : 0};
+    return AB_OK;
+}
+
+// The tuning table's entry for a batch of nq queries, applied (settings that
+// change bump the launch generation).
+static void apply_batch_tuning(ab_ctx *c, uint32_t nq)
+{
+    const ab_tuning *pick = nullptr;
+    for (const ab_tuning &t : c->tuning)
+        if (t.min_batch <= nq) pick = &t;
+    if (!pick) return;
+    const bool col = pick->dense_columns != 0;
+    if (c->group == pick->group && c->dense_threads == pick->dense_threads &&
+        c->sparse_chunk == pick->sparse_chunk && c->dense_columns == col)
+        return;
+    c->group = pick->group;
+    c->dense_threads = pick->dense_threads;
+    c->sparse_chunk = pick->sparse_chunk;
+    c->dense_columns = col;
+    c->cfg_gen++;
+}
+
 extern "C" ab_status ab_ctx_set_batch_mode(ab_ctx *c, int mode)
 {
     if (!c || mode < 0 || mode > 2) {
@@ -1224,6 +1276,7 @@ static ab_status query_batch(const char *fn, ab_ctx *c, const uint32_t *d_keys, 
             set_error(std::string(fn) + ": offsets must be non-decreasing");
             return AB_INVALID_ARG;
         }
+    apply_batch_tuning(c, nq);
     BatchArgs a;
     a.d_keys = d_keys;
     a.h_qoff = h_qoff;

@@ -90,6 +90,8 @@ struct ab_ctx {
     int batch_mode = 2;
     uint64_t cfg_gen = 1;  // bumped by every setting that changes launch parameters
     ab::DevBuf qgroup, grp_q, grp_n, plan, counter, gmask, ganybm;
+    // launch settings by batch size (ab_ctx_set_batch_tuning), ascending min_batch
+    std::vector<ab_tuning> tuning;
     // id-only batches (ab_query_batch_ids): values expanded per part of at
     // most id_budget values, chunk counts and bases, sort output, run starts,
     // rocPRIM temporary storage

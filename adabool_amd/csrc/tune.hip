@@ -70,6 +70,14 @@ extern "C" ab_status ab_ctx_autotune(ab_ctx *c, const uint32_t *d_keys, const ui
         return AB_DEVICE_ERROR;
     }
     const uint32_t saved[4] = {c->group, c->dense_threads, c->sparse_chunk, (uint32_t)c->dense_columns};
+    // the candidates' settings must hold for every call: the batch-size table
+    // is set aside until the end
+    struct KeepTable {
+        ab_ctx *c;
+        std::vector<ab_tuning> t;
+        explicit KeepTable(ab_ctx *cc) : c(cc) { t.swap(c->tuning); }
+        ~KeepTable() { c->tuning.swap(t); }
+    } keep(c);
     // result bitmaps of the entry settings: every candidate must reproduce them
     const uint64_t bm_bytes = (uint64_t)nq * c->ix->stride;  // stride: a multiple of 16 bytes
     TuneScratch sc;

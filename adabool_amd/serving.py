@@ -89,22 +89,27 @@ def main(argv=None):
             serve_ix = Index.build_device(dk, dv, spec.nkeys, spec.nvalues, ts)
         log(f"settings as given: {tuned}")
     elif not a.no_autotune:
-        ns = min(nq, 256)
-        sk = torch.from_numpy(np.ascontiguousarray(qk[: int(qo[ns])]).view(np.int32)).cuda()
-        so = qo[: ns + 1]
+        # launch settings by batch size: samples of 32 and 256 queries (light
+        # and medium load) and of max_batch (saturation; it also picks the
+        # value tile), each applied from the geometric mean of its sample size
+        # and the one below
+        def sample(ns):
+            ns = min(nq, ns)
+            return torch.from_numpy(np.ascontiguousarray(qk[: int(qo[ns])]).view(np.int32)).cuda(), qo[: ns + 1]
 
-        def tune_on(index):
+        def tune_on(index, s):
             c = Context(index)
             if os.path.exists(a.model):
                 c.load_model(a.model)
-            r = c.autotune(sk, so, AB_PATH_AUTO, repeats=2)[0]
+            r = c.autotune(s[0], s[1], AB_PATH_AUTO, repeats=2)[0]
             c.close()
             return r
 
-        best = dict(tune_on(ix), tile_shift=info["tile_shift"])
+        big, mid, small = sample(a.max_batch), sample(256), sample(32)
+        best = dict(tune_on(ix, big), tile_shift=info["tile_shift"])
         for ts in [int(x) for x in a.tile_shifts.split(",") if x and int(x) != info["tile_shift"]]:
             cand = Index.build_device(dk, dv, spec.nkeys, spec.nvalues, ts)
-            r = tune_on(cand)
+            r = tune_on(cand, big)
             log(f"autotune tile 2^{ts}: {r['ms']:.3f} ms vs {best['ms']:.3f} ms")
             if r["ms"] < best["ms"]:
                 if serve_ix is not ix:
@@ -112,7 +117,15 @@ def main(argv=None):
                 serve_ix, best = cand, dict(r, tile_shift=ts)
             else:
                 cand.close()
-        tuned = best
+        keys = ("group", "dense_threads", "sparse_chunk", "dense_columns")
+        table, lo = [], 0
+        for s, r in ((small, tune_on(serve_ix, small)), (mid, tune_on(serve_ix, mid)), (big, best)):
+            n = int(s[1].size - 1)
+            start = 1 if not table else max(int(round((lo * n) ** 0.5)), table[-1]["min_batch"] + 1)
+            if not table or n > lo:
+                table.append(dict({k: r[k] for k in keys}, min_batch=start, sample=n, ms=r["ms"]))
+            lo = n
+        tuned = {"tile_shift": best["tile_shift"], "table": table}
         log(f"autotune: {tuned}")
     del dk, dv, pk, pv
     torch.cuda.empty_cache()
@@ -163,7 +176,10 @@ def main(argv=None):
         srv.load_model(a.model)
     if tuned:
         for c in srv.contexts:
-            c.apply_tuning(tuned)
+            if "table" in tuned:
+                c.set_batch_tuning(tuned["table"])
+            else:
+                c.apply_tuning(tuned)
     # warm-up (allocations, code objects)
     srv.replay(qk[: int(qo[min(nq, 2048)])], qo[: min(nq, 2048) + 1], 0.0, 7)
     srv.stats(reset=True)

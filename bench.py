@@ -244,12 +244,21 @@ def c3_leg(ab, torch, ctx, ix, spec, pk, path, steps: int = 5, nq: int = 4096):
     return out, (qk, qo, counts)
 
 
+# Serving launch settings by batch size on the C5 filter (value tile 2^10):
+# 64-query groups and 256 threads keep light and medium load (batches of up
+# to a few hundred queries) fastest, 256-query groups and 512 threads the
+# near-full batches of saturation.
+C5_TUNING = [{"min_batch": 1, "group": 64, "dense_threads": 256, "sparse_chunk": 4096, "dense_columns": True},
+             {"min_batch": 512, "group": 256, "dense_threads": 512, "sparse_chunk": 4096, "dense_columns": True}]
+
+
 def c5_leg(ab, torch, model_path: str, nq: int = 8192, rates=(0.0, 10000.0, 50000.0)):
     """C5 serving: the 16M x 4M filter (~400M pairs) built on the device and
     resident, C3-mix requests replayed through the serving loop (1024-query
-    batches, two in flight, memory-based admission into a 64 GiB result
-    arena, learned dispatch): saturation (all at once) and Poisson arrivals
-    at 10k and 50k queries/s, with p50/p99 latency from each arrival."""
+    batches, two in flight, memory-based admission into a 128 GiB result
+    arena, learned dispatch, launch settings by batch size): saturation (all
+    at once) and Poisson arrivals at 10k and 50k queries/s, with p50/p99
+    latency from each arrival."""
     from adabool_amd import configs
 
     spec = configs.C5
@@ -265,15 +274,13 @@ def c5_leg(ab, torch, model_path: str, nq: int = 8192, rates=(0.0, 10000.0, 5000
     info = ix.info
     qk, qo = configs.make_queries(spec, configs.C3_Q, nq)
     log(f"C5 index: {info['nnz']} pairs, device {info['device_bytes'] / 1e9:.2f} GB ({time.time() - t:.1f}s)")
-    srv = ab.Server(ix, max_batch=1024, max_wait_us=200, inflight=2, arena_bytes=64 << 30, batch_mode=1)
+    srv = ab.Server(ix, max_batch=1024, max_wait_us=200, inflight=2, arena_bytes=128 << 30, batch_mode=1)
     runs = []
     try:
         if os.path.exists(model_path):
             srv.load_model(model_path)
-        for c in srv.contexts:  # the best of a sweep of these settings on this filter
-            c.set_dense_group(128)
-            c.set_dense_threads(512)
-            c.set_sparse_chunk(8192)
+        for c in srv.contexts:  # by batch size, from a sweep of settings on this filter
+            c.set_batch_tuning(C5_TUNING)
         srv.replay(qk[: int(qo[2048])], qo[:2049], 0.0, 7)
         srv.stats(reset=True)
         ref = None
@@ -682,6 +689,10 @@ def main():
         "c3_mix_batch4096": c3,
     }
     if not args.no_c5 and args.scale == 1.0:
+        # the C2 steps' outputs (about 33 GB) make room for the result arena
+        bufs.clear()
+        d_bm = d_ids = d_cnt = sb = si = None
+        torch.cuda.empty_cache()
         out["c5_serving"] = c5_leg(ab, torch, args.model)
     print(json.dumps(out), flush=True)
 

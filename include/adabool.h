@@ -174,6 +174,26 @@ ab_status ab_ctx_set_dense_columns(ab_ctx *ctx, int on);
 /* Records per work item of the sparse kernel (power of two, 256..8192). */
 ab_status ab_ctx_set_sparse_chunk(ab_ctx *ctx, uint32_t items);
 
+/* Launch settings by batch size: every ab_query_batch / ab_query_batch_packed
+ * call of nq queries first applies the entry with the largest min_batch <= nq
+ * (the four settings above; none below the first entry's min_batch).  Entries
+ * in strictly ascending min_batch (>= 1); n = 0 removes the table.  A serving
+ * loop whose batches range from a few queries to its maximum uses one table
+ * instead of settings tuned for one size (ab_ctx_autotune at each size gives
+ * the entries; the autotuner itself ignores the table).  No reference
+ * counterpart (the reference's launch parameters are compile-time constants). */
+typedef struct ab_tuning {
+    uint32_t min_batch;
+    uint32_t group;          /* ab_ctx_set_dense_group */
+    uint32_t dense_threads;  /* ab_ctx_set_dense_threads */
+    uint32_t sparse_chunk;   /* ab_ctx_set_sparse_chunk */
+    int32_t dense_columns;   /* ab_ctx_set_dense_columns */
+} ab_tuning;
+ab_status ab_ctx_set_batch_tuning(ab_ctx *ctx, const ab_tuning *entries, uint32_t n);
+/* The launch settings in force (those the last batch applied, or as set);
+ * out->min_batch = 0. */
+ab_status ab_ctx_get_batch_settings(const ab_ctx *ctx, ab_tuning *out);
+
 /* How ab_query_batch runs a batch: 0 host dispatch (features and decisions
  * copied back, one launch per dense group), 1 device dispatch (decisions,
  * grouping and masks on the device, then ONE persistent launch that takes

@@ -249,3 +249,42 @@ def test_autotune_winner_matches_oracle(c1):
     _verify(csr, qk, qo, ctx2.query_batch(qk, qo, ab.AB_PATH_THRESHOLD), f"applied {best}")
     ctx2.close()
     ctx.close()
+
+
+def test_batch_tuning_table(c1):
+    """ab_ctx_set_batch_tuning: each batch applies the entry for its size (a
+    batch below the first entry keeps the settings in force), results equal
+    the oracle's at every switch in both device modes, bad tables are refused,
+    and the autotuner's candidates are not overridden by the table."""
+    ix, csr = c1
+    small = {"min_batch": 2, "group": 8, "dense_threads": 256, "sparse_chunk": 256, "dense_columns": False}
+    large = {"min_batch": 40, "group": 128, "dense_threads": 512, "sparse_chunk": 8192, "dense_columns": True}
+    for mode in (ab.Context.BATCH_FUSED, ab.Context.BATCH_GRAPH):
+        ctx = ab.Context(ix)
+        ctx.set_batch_mode(mode)
+        ctx.set_batch_threshold(0.01)
+        ctx.set_batch_tuning([small, large])
+        for nq, want in ((12, small), (60, large), (1, large), (39, small), (150, large)):
+            qk, qo = _mixed(0xADAB0208 + nq, nq, configs.C1.nkeys)
+            print(f"tuning table mode={mode} nq={nq}", flush=True)
+            res = ctx.query_batch(qk, qo, ab.AB_PATH_THRESHOLD)
+            got = ctx.batch_settings()
+            assert got == {k: v for k, v in want.items() if k != "min_batch"}, (nq, got)
+            _verify(csr, qk, qo, res, f"table mode {mode} nq {nq}")
+        ctx.close()
+    ctx = ab.Context(ix)
+    with pytest.raises(ab.AdaBoolError):
+        ctx.set_batch_tuning([large, small])  # not ascending
+    with pytest.raises(ab.AdaBoolError):
+        ctx.set_batch_tuning([dict(small, dense_threads=100)])
+    ctx.set_batch_tuning([dict(large, min_batch=1)])
+    qk, qo = _mixed(0xADAB0209, 24, configs.C1.nkeys)
+    import torch
+
+    d_keys = torch.from_numpy(qk.view(np.int32)).cuda()
+    res = ctx.autotune(d_keys, qo, ab.AB_PATH_THRESHOLD, repeats=1)
+    # each candidate ran with its own group width (with the table in force
+    # every one would have recorded the entry's)
+    assert len({r["group"] for r in res}) > 1, res
+    ctx.set_batch_tuning([])
+    ctx.close()
