@@ -190,7 +190,9 @@ def c3_leg(ab, torch, ctx, ix, spec, pk, path, steps: int = 5, nq: int = 4096):
     Zipf-weighted keys) per step through the same autotuned context (graph
     mode), ids packed at offsets from each query's id bound.  Steps run one
     at a time; the union's roofline counts each query's list bytes and its
-    bitmap (no sharing credit) over the sparse route + fused union time."""
+    bitmap (no sharing credit) over the sparse route + fused union time.  The
+    same batch then runs under the threshold rule and with every query dense
+    (other_dispatch), with counts equal to the first rule's."""
     from adabool_amd import configs
 
     qk, qo = configs.make_queries(spec, configs.C3_Q, nq)
@@ -225,6 +227,21 @@ def c3_leg(ab, torch, ctx, ix, spec, pk, path, steps: int = 5, nq: int = 4096):
     ctx.sync()
     kt = ctx.timing(reset=True)
     ctx.set_timing(False)
+    # the same batch under the other dispatch rules: a mix where the choice
+    # matters (same counts on every rule)
+    rules = {}
+    for name, p in (("threshold", ab.AB_PATH_THRESHOLD), ("all_dense", ab.AB_PATH_DENSE)):
+        for _ in range(2):
+            pp = ctx.query_batch_packed(d_keys, qo, p, d_ids, off, None, d_cnt)
+        ctx.sync()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            ctx.query_batch_packed(d_keys, qo, p, d_ids, off, None, d_cnt)
+            ctx.sync()
+        rules[name] = {"qps": round(nq * 3 / (time.perf_counter() - t1), 1),
+                       "dense_queries": int((pp == ab.AB_PATH_DENSE).sum()),
+                       "counts_equal": bool(np.array_equal(d_cnt.cpu().numpy().view(np.uint32), counts))}
+        log(f"C3 mix batch {nq}, {name} dispatch: {rules[name]['qps']:.0f} q/s, dense {rules[name]['dense_queries']}")
     union_ms = (kt["fused_union"][0] + kt["sparse_union"][0]) / 2
     alg = float((list_bytes_per_query(ix, qk, qo) + ix.stride_bytes).sum())
     out = {"workload": f"C2 filter, {nq} C3-mix queries (power law 1.5 on [1, 50000], Zipf-weighted keys) per "
@@ -233,6 +250,7 @@ def c3_leg(ab, torch, ctx, ix, spec, pk, path, steps: int = 5, nq: int = 4096):
            "batch_p99_ms": float(np.percentile(lat, 99)), "dense_queries": int((paths == ab.AB_PATH_DENSE).sum()),
            "ids_per_query_mean": float(counts.mean()),
            "kernel_ms_per_step": {k: round(v[0] / 2, 4) for k, v in kt.items() if v[1]},
+           "other_dispatch": rules,
            "union_roofline": {"bound": "hbm", "achieved": round(alg / (union_ms * 1e-3) / 1e9, 1),
                               "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(alg / (union_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
