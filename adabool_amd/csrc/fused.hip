@@ -78,6 +78,9 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     const uint32_t nq = p.nq;
     const uint64_t *qoff = (const uint64_t *)c->qoff.p;
     uint8_t *sel = (uint8_t *)c->sel.p;
+    // the union's slice items count their queries' compaction chunks, so the
+    // counting pass reads only the other queries' bitmaps
+    const bool slice_counts = (a.d_ids || a.d_counts) && p.sp_cap && p.words;
     // qoff and, for packed ids, the id offsets arrive in one copy
     HIP_TRY(hipMemcpyAsync(c->qoff.p, c->hstage.p, p.stage_bytes, hipMemcpyHostToDevice, s));
     const uint64_t *Kd = p.graph ? qoff + nq : nullptr;  // qoff[nq] = the key count
@@ -181,6 +184,10 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
         f.out_words = p.words;
         f.use_col = p.use_col ? 1 : 0;
         f.counter = (uint32_t *)c->counter.p;
+        if (slice_counts) {
+            f.ccnt = (uint32_t *)c->ccnt.p;
+            f.nchunk = (uint32_t)((p.words + kCompactWords - 1) / kCompactWords);
+        }
         HIP_TRY(launch_union_fused(p.mode, c->dense_threads, d, f, p.G, p.grid, s));
     }
     {
@@ -192,7 +199,7 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
     if (a.d_ids || a.d_counts) {
         {
             TimedSpan ts(c, AB_K_POPCOUNT);
-            launch_popcount(p.bm, p.stride, p.words, nq, (uint32_t *)c->ccnt.p, s);
+            launch_popcount(p.bm, p.stride, p.words, nq, (uint32_t *)c->ccnt.p, s, slice_counts ? sa.flag : nullptr);
         }
         TimedSpan ts(c, AB_K_WRITE_IDS);
         const uint64_t *d_idoff = p.packed ? qoff + (nq + 1) : nullptr;

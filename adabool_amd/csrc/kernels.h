@@ -149,6 +149,11 @@ struct FusedArgs {
     const uint64_t *sp_rbase;
     const uint16_t *sp_choff;
     uint32_t sp_chunk, nslices;
+    // compaction chunk counts of the slice queries (ccnt NULL: none): each
+    // slice item writes its query's counts of the chunks it covers to
+    // ccnt[q * nchunk + chunk]
+    uint32_t *ccnt;
+    uint32_t nchunk;
 };
 uint32_t fused_lds_bytes(int mode, uint32_t G, uint32_t tile_shift, uint32_t threads, bool use_col);
 // Work-groups of the persistent union launch: CUs x resident work-groups per
@@ -158,8 +163,10 @@ hipError_t fused_grid(int mode, uint32_t threads, const DevIndex &ix, uint32_t G
 hipError_t launch_union_fused(int mode, uint32_t threads, const DevIndex &ix, const FusedArgs &a, uint32_t G,
                               uint32_t grid, hipStream_t s);
 constexpr uint32_t kCompactWords = 2048;
+// skip (when not NULL): queries with skip[q] == kSpSlice are left alone (the
+// union's slice items counted them).
 void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq,
-                     uint32_t *ccnt, hipStream_t s);
+                     uint32_t *ccnt, hipStream_t s, const uint8_t *skip = nullptr);
 // Query q's ids go to ids + id_off[q] (device offsets) or, with id_off NULL,
 // to ids + q * ids_stride; off_aligned: every id_off[q] is a multiple of 4.
 // ccnt (from launch_popcount) is turned into per-query exclusive prefixes in

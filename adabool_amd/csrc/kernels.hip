@@ -1290,6 +1290,24 @@ __device__ __forceinline__ void sparse_emit_slice(const DevIndex &ix, const Fuse
     or_slice_refs<NT>(ix, co, refs, nch, S, a.sp_chunk, s, (uint32_t)w0, slice, s_lo, s_pre, s_sh);
     if (w0 >= a.out_words) return;  // block-uniform
     const uint64_t nw = a.out_words - w0 < kSpSliceWords ? a.out_words - w0 : kSpSliceWords;
+    if (a.ccnt) {
+        // the slice's compaction chunk counts (words past the bitmap are zero)
+        constexpr uint32_t CPS = kSpSliceWords / kCompactWords;
+        __shared__ uint32_t s_part[CPS][NT / 64];
+        for (uint32_t j = 0; j < CPS; ++j) {
+            uint32_t n = 0;
+            for (uint32_t i = threadIdx.x; i < kCompactWords; i += NT) n += __popc(slice[j * kCompactWords + i]);
+            n = wave_sum(n);
+            if (lane_id() == 0) s_part[j][threadIdx.x >> 6] = n;
+        }
+        __syncthreads();
+        if (threadIdx.x < CPS) {
+            const uint32_t c = s * CPS + threadIdx.x;
+            uint32_t n = 0;
+            for (uint32_t w = 0; w < NT / 64; ++w) n += s_part[threadIdx.x][w];
+            if (c < a.nchunk) a.ccnt[(uint64_t)q * a.nchunk + c] = n;
+        }
+    }
     uint32_t *dst = a.out + (uint64_t)q * a.out_stride + w0;
     if ((a.out_stride & 3) == 0 && (nw & 3) == 0) {
         const uint4 *src = reinterpret_cast<const uint4 *>(slice);
@@ -1308,8 +1326,11 @@ __global__ void k_set_u32(uint32_t *p, uint32_t v)
 // dense (group, tile) items, tile by tile, then the (slice query, slice)
 // emit items of the write-once sparse work -- until none are left.  Dense
 // items come first: they are the long ones.
+// Two 512-thread work-groups per CU need at most 128 VGPRs (4 waves per
+// SIMD); past that only one fits and the union slows by half.
 template <int MODE, int NT>
-__global__ __launch_bounds__(NT) void k_union_fused(DevIndex ix, FusedArgs a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 1)))
+void k_union_fused(DevIndex ix, FusedArgs a)
 {
     using M = typename MaskOf<MODE>::T;
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
@@ -1406,10 +1427,11 @@ hipError_t launch_union_fused(int mode, uint32_t threads, const DevIndex &ix, co
 // bitmap -> ascending ids (CollectIdsFromBitmap, test/adsfilter_q_all.cu:108-119)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_popcount(const uint32_t *__restrict__ bm, uint64_t stride, uint64_t words,
-                                                  uint32_t nchunk, uint32_t *ccnt)
+                                                  uint32_t nchunk, uint32_t *ccnt, const uint8_t *skip)
 {
     __shared__ uint32_t sh[4];
     const uint32_t q = blockIdx.y, c = blockIdx.x;
+    if (skip && skip[q] == kSpSlice) return;
     const uint64_t w0 = (uint64_t)c * kCompactWords;
     const uint64_t w1 = w0 + kCompactWords < words ? w0 + kCompactWords : words;
     const uint32_t *src = bm + (uint64_t)q * stride;
@@ -1421,14 +1443,15 @@ __global__ __launch_bounds__(256) void k_popcount(const uint32_t *__restrict__ b
     if (threadIdx.x == 0) ccnt[(uint64_t)q * nchunk + c] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, uint32_t *ccnt, hipStream_t s)
+void launch_popcount(const uint32_t *bm, uint64_t stride, uint64_t words, uint32_t nq, uint32_t *ccnt, hipStream_t s,
+                     const uint8_t *skip)
 {
     if (!nq || !words) return;
     uint32_t nchunk = (uint32_t)((words + kCompactWords - 1) / kCompactWords);
     for (uint32_t q0 = 0; q0 < nq; q0 += kMaxGridY) {
         const uint32_t n = nq - q0 < kMaxGridY ? nq - q0 : kMaxGridY;
         hipLaunchKernelGGL(k_popcount, dim3(nchunk, n), dim3(256), 0, s, bm + (uint64_t)q0 * stride, stride, words,
-                           nchunk, ccnt + (uint64_t)q0 * nchunk);
+                           nchunk, ccnt + (uint64_t)q0 * nchunk, skip ? skip + q0 : nullptr);
     }
 }
 
