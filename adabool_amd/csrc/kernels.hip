@@ -1436,7 +1436,19 @@ __global__ __launch_bounds__(256) void k_popcount(const uint32_t *__restrict__ b
     const uint64_t w1 = w0 + kCompactWords < words ? w0 + kCompactWords : words;
     const uint32_t *src = bm + (uint64_t)q * stride;
     uint32_t s = 0;
-    for (uint64_t i = w0 + threadIdx.x; i < w1; i += 256) s += __popc(src[i]);
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        // 16-byte loads over the chunk's whole quads (w0 is a multiple of 4),
+        // then the words past the last one
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+        const uint64_t e4 = w1 / 4;
+        for (uint64_t i = w0 / 4 + threadIdx.x; i < e4; i += 256) {
+            const uint4 v = s4[i];
+            s += __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+        }
+        for (uint64_t i = e4 * 4 + threadIdx.x; i < w1; i += 256) s += __popc(src[i]);
+    } else {
+        for (uint64_t i = w0 + threadIdx.x; i < w1; i += 256) s += __popc(src[i]);
+    }
     s = wave_sum(s);
     if (lane_id() == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
