@@ -27,6 +27,25 @@ from . import AB_PATH_AUTO, AB_PATH_THRESHOLD, Context, Index, Server, configs, 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+TUNING_KEYS = ("group", "dense_threads", "sparse_chunk", "dense_columns")
+
+
+def tuning_table(results):
+    """Launch settings by batch size (Context.set_batch_tuning) from autotune
+    winners on samples of ascending size: [(sample queries, winner dict)].
+    Each winner applies from the geometric mean of its sample size and the
+    previous one (the first from batches of 1); a sample no larger than the
+    previous one adds nothing."""
+    table, lo = [], 0
+    for n, r in results:
+        if table and n <= lo:
+            continue
+        start = 1 if not table else max(int(round((lo * n) ** 0.5)), table[-1]["min_batch"] + 1)
+        table.append(dict({k: r[k] for k in TUNING_KEYS}, min_batch=start, sample=int(n), ms=r["ms"]))
+        lo = n
+    return table
+
+
 def log(msg):
     print(f"[serving] {msg}", file=sys.stderr, flush=True)
 
@@ -117,15 +136,12 @@ def main(argv=None):
                 serve_ix, best = cand, dict(r, tile_shift=ts)
             else:
                 cand.close()
-        keys = ("group", "dense_threads", "sparse_chunk", "dense_columns")
-        table, lo = [], 0
-        for s, r in ((small, tune_on(serve_ix, small)), (mid, tune_on(serve_ix, mid)), (big, best)):
-            n = int(s[1].size - 1)
-            start = 1 if not table else max(int(round((lo * n) ** 0.5)), table[-1]["min_batch"] + 1)
-            if not table or n > lo:
-                table.append(dict({k: r[k] for k in keys}, min_batch=start, sample=n, ms=r["ms"]))
-            lo = n
-        tuned = {"tile_shift": best["tile_shift"], "table": table}
+        sizes = [int(x[1].size - 1) for x in (small, mid, big)]
+        results = [(sizes[0], tune_on(serve_ix, small))]
+        if sizes[1] > sizes[0]:
+            results.append((sizes[1], tune_on(serve_ix, mid)))
+        results.append((sizes[2], best))
+        tuned = {"tile_shift": best["tile_shift"], "table": tuning_table(results)}
         log(f"autotune: {tuned}")
     del dk, dv, pk, pv
     torch.cuda.empty_cache()
