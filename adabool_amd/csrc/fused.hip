@@ -184,6 +184,7 @@ ab_status enqueue(ab_ctx *c, const BatchArgs &a, const Plan &p)
         f.out_words = p.words;
         f.use_col = p.use_col ? 1 : 0;
         f.counter = (uint32_t *)c->counter.p;
+        f.ngroups_cap = p.ngroups;
         if (slice_counts) {
             f.ccnt = (uint32_t *)c->ccnt.p;
             f.nchunk = (uint32_t)((p.words + kCompactWords - 1) / kCompactWords);
@@ -267,7 +268,7 @@ ab_status run_batch_fused(ab_ctx *c, const BatchArgs &a)
     AB_TRY(c->grp_q.ensure((size_t)ngmax * kGroupSlots * 4));
     AB_TRY(c->grp_n.ensure((size_t)ngmax * 4 + 4));
     AB_TRY(c->plan.ensure(16));
-    AB_TRY(c->counter.ensure(16));
+    AB_TRY(c->counter.ensure(((size_t)ngmax + 2) * 4));  // slice items, then one per dense group
     AB_TRY(c->kq.ensure(p.Kcap * 4 + 4));
     AB_TRY(c->cnt.ensure(p.Kcap * 8 + 8));
     AB_TRY(c->koff.ensure((p.Kcap + 1) * 8));

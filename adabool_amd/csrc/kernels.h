@@ -143,7 +143,10 @@ struct FusedArgs {
     uint32_t *out;
     uint64_t out_stride, out_words;
     int use_col;
-    uint32_t *counter;           // work counter (reset by the launcher)
+    // work counters, reset by the launcher: [0] the slice items, [1 + g] the
+    // tiles of dense group g (ngroups_cap of them)
+    uint32_t *counter;
+    uint32_t ngroups_cap;
     // write-once sparse: one emit item per (slice query, slice), after the dense items
     const uint32_t *sp_tot, *sp_slist, *sp_cpre, *sp_scpre, *sp_refs;
     const uint64_t *sp_rbase;
