@@ -76,11 +76,24 @@ inline uint32_t grid_for(uint64_t n, uint32_t bs = 256)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 1u << 20));
 }
 
-// largest key and value; one atomic per wave
-__global__ void k_max_pairs(const uint32_t *k, const uint32_t *v, uint64_t n, uint32_t *out)
+// Largest key and value. A bounded grid strides over the pairs, and each
+// 256-thread work-group makes one pair of atomics. With a work-group per 256
+// pairs, two same-address atomics per wave serialised at the L2: 95 ms for
+// 4e8 pairs, against about 1 ms of reading.
+constexpr uint32_t kMaxPairsGrid = 2048;
+__global__ void __launch_bounds__(256) k_max_pairs(const uint32_t *k, const uint32_t *v, uint64_t n, uint32_t *out)
 {
+    __shared__ uint32_t s_ab[2][4];
     uint32_t a = 0, b = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256;
+    uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const uint32_t k0 = k[i], k1 = k[i + stride], k2 = k[i + 2 * stride], k3 = k[i + 3 * stride];
+        const uint32_t v0 = v[i], v1 = v[i + stride], v2 = v[i + 2 * stride], v3 = v[i + 3 * stride];
+        a = max(a, max(max(k0, k1), max(k2, k3)));
+        b = max(b, max(max(v0, v1), max(v2, v3)));
+    }
+    for (; i < n; i += stride) {
         a = max(a, k[i]);
         b = max(b, v[i]);
     }
@@ -90,6 +103,13 @@ __global__ void k_max_pairs(const uint32_t *k, const uint32_t *v, uint64_t n, ui
         b = max(b, (uint32_t)__shfl_xor(b, d, 64));
     }
     if ((threadIdx.x & 63) == 0) {
+        s_ab[0][threadIdx.x >> 6] = a;
+        s_ab[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = max(max(s_ab[0][0], s_ab[0][1]), max(s_ab[0][2], s_ab[0][3]));
+        b = max(max(s_ab[1][0], s_ab[1][1]), max(s_ab[1][2], s_ab[1][3]));
         atomicMax(&out[0], a);
         atomicMax(&out[1], b);
     }
@@ -348,7 +368,8 @@ extern "C" ab_status ab_index_build_device(const uint32_t *d_keys, const uint32_
         Tmp mx;
         HIP_TRY(mx.alloc(8));
         HIP_TRY(hipMemsetAsync(mx.p, 0, 8, s));
-        hipLaunchKernelGGL(k_max_pairs, dim3(grid_for(nnz)), dim3(256), 0, s, d_keys, d_vals, nnz, mx.as<uint32_t>());
+        hipLaunchKernelGGL(k_max_pairs, dim3(std::min(grid_for(nnz), kMaxPairsGrid)), dim3(256), 0, s, d_keys, d_vals,
+                           nnz, mx.as<uint32_t>());
         uint32_t h[2];
         HIP_TRY(hipMemcpyAsync(h, mx.p, 8, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));

@@ -231,3 +231,24 @@ def test_device_build_is_byte_identical(tmp_path, case):
     host.save(ph)
     dev.save(pd)
     assert open(ph, "rb").read() == open(pd, "rb").read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,where", [(1, "first"), (1000, "last"), (3_000_001, "first"), (3_000_001, "last"),
+                                     (3_000_001, "middle")])
+def test_device_build_infers_spaces(n, where):
+    """With nkeys = nvalues = 0 the device build takes max+1 of the pairs, as
+    the host build does. The sizes cover the strided loop's unrolled body and
+    its tail; the largest key and value sit at the first, last or a middle
+    pair, in different places for keys and values."""
+    rng = np.random.default_rng(n + len(where))
+    k = rng.integers(0, 5000, n).astype(np.uint32)
+    v = rng.integers(0, 90_000, n).astype(np.uint32)
+    pos = {"first": 0, "last": n - 1, "middle": n // 2 + 7}[where]
+    k[pos] = 77_777
+    v[(pos + n // 3) % n] = 1_234_567
+    print(f"device build infers spaces n={n} {where}", flush=True)
+    host = ab.Index(k, v, 0, 0, host_only=True)
+    dev = ab.Index.build_device(k, v, 0, 0)
+    assert dev.info["nkeys"] == host.info["nkeys"] == int(k.max()) + 1
+    assert dev.info["nvalues"] == host.info["nvalues"] == int(v.max()) + 1
